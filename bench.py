@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Benchmark: effective GB/s of the speculative segmented-sum CSR SpMV on one
+MI355X, on the power-law workload (m = n = 2^20, P(k) ~ k^-2 on [1, 1e5],
+10% empty rows, ~10M nonzeros, values and x uniform in [-1, 1], FP64).
+
+One step = one y = A*x (main kernel + calibration kernel) with A and x already
+resident in HBM.  L2 and the MALL are flushed before every step (a 1 GiB
+write outside the timed events), so every step starts cold.  Steps are timed
+with HIP events on the handle's stream inside libbhsparse_spmv.so; the whole
+timed loop is also bracketed by device synchronisation and host wall clock.
+
+Effective bytes = nnz*(sizeof(val)+4) + 4*(m+1) + n*sizeof(val) + m*sizeof(val).
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from benchmark_spmv_using_csr_amd import _lib, matgen  # noqa: E402
+from benchmark_spmv_using_csr_amd.spmv import bhsparse_spmv, device_synchronize  # noqa: E402
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "effective GB/s (% HBM peak) CSR SpMV; tuned+batched vs plain speedup at equal err"
+
+
+def eff_bytes(m, n, nnz, vsize, k=1):
+    return nnz * (vsize + 4) + 4 * (m + 1) + k * (n + m) * vsize
+
+
+def load_traffic(name):
+    """HBM bytes per launch of the main kernel from the committed rocprofv3
+    --pmc summary for this workload (FETCH_SIZE + WRITE_SIZE, scaled by the copy
+    kernel calibration recorded with it); None if absent."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            d = json.load(f)
+        return float(d["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def run_gpu(A, x, dtype, cfg, warmup, steps, flush_bytes):
+    op = bhsparse_spmv(dtype)
+    _lib.check(op.init_platform(0), "init_platform")
+    _lib.check(op.set_config(0, *cfg), "set_config")
+    y = np.zeros(A.m, dtype)
+    t0 = time.perf_counter()
+    _lib.check(op.prepare_mem(A.m, A.n, A.nnz, A.row_ptr, A.col, A.val.astype(dtype),
+                              x.astype(dtype), y), "prepare_mem")
+    t_init = time.perf_counter() - t0
+    device_synchronize()
+    w0 = time.perf_counter()
+    ms_total, ms_main = op.bench(1.0, warmup=warmup, reps=steps, flush_bytes=flush_bytes)
+    device_synchronize()
+    wall = time.perf_counter() - w0
+    _lib.check(op.get_y(), "get_y")
+    # warm numbers (no flush) for reference
+    wt, wm = op.bench(1.0, warmup=3, reps=max(10, steps // 2), flush_bytes=0)
+    op.close()
+    return {
+        "ms_total": ms_total, "ms_main": ms_main, "wall_s": wall, "init_s": t_init,
+        "warm_total": wt, "warm_main": wm, "y": y,
+    }
+
+
+def cpu_baseline(A, x, dtype, y_gpu, budget_s=12.0):
+    """The reference's CPU paths timed on this host: the serial loop
+    (main.cu:755-761) and the OpenMP row-parallel loop (main.cu:377-393) on all
+    host cores, in the same precision and byte formula.  Also checks the GPU
+    result against the FP64 serial oracle."""
+    import oracle  # the CPU baseline leg is the only place bench.py loads the oracle
+
+    v = A.val.astype(dtype)
+    xx = x.astype(dtype)
+    s1, _ = oracle.time_spmv(A.row_ptr, A.col, v, xx, 1, 3)
+    ncores = os.cpu_count() or 1
+    try:
+        ncores = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    so, used = oracle.time_spmv(A.row_ptr, A.col, v, xx, ncores, 5)
+    reps = int(max(5, min(400, budget_s / max(so, 1e-4))))
+    so, used = oracle.time_spmv(A.row_ptr, A.col, v, xx, ncores, reps)
+    b = eff_bytes(A.m, A.n, A.nnz, np.dtype(dtype).itemsize)
+    v64, x64 = v.astype(np.float64), xx.astype(np.float64)
+    yr = oracle.spmv_serial(A.row_ptr, A.col, v64, x64)
+    err = oracle.rel_error(A.row_ptr, A.col, v64, x64, y_gpu, yr)
+    return {
+        "value": round(b / so / 1e9, 3), "unit": "GB/s", "cores": used, "kind": "port",
+        "sample": f"{A.name}, {np.dtype(dtype).name}: OpenMP row loop (main.cu:377-393), "
+                  f"median of {reps} calls on {used} threads; serial loop "
+                  f"{b / s1 / 1e9:.3f} GB/s (median of 3 calls, 1 core)",
+        "serial_gbps": round(b / s1 / 1e9, 3),
+        "gpu_max_rel_err_vs_fp64_oracle": err,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--dtype", choices=["f64", "f32"], default="f64")
+    ap.add_argument("--items-per-lane", type=int, default=0)
+    ap.add_argument("--waves-per-block", type=int, default=4)
+    ap.add_argument("--tiles-per-wave", type=int, default=0)
+    ap.add_argument("--flush-mb", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=["power-law", "uniform", "laplace", "block3"],
+                    default="power-law")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported "
+              "(the multi-GPU nnz sharding is a design note, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    dtype = np.float64 if a.dtype == "f64" else np.float32
+    S = a.items_per_lane or (8 if dtype == np.float64 else 16)
+    tpw = a.tiles_per_wave or (3 if dtype == np.float64 else 2)
+    cfg = (S, a.waves_per_block, tpw, 0)
+    if a.workload == "power-law":
+        A, x = matgen.power_law_1m(), matgen.vector(102, 1 << 20)
+    elif a.workload == "uniform":
+        A, x = matgen.uniform_65536(), matgen.vector(101, 65536)
+    elif a.workload == "laplace":
+        A = matgen.laplacian_160()
+        x = matgen.vector(103, A.n)
+    else:
+        A = matgen.block3_2m()
+        x = matgen.vector(104, A.n)
+
+    r = run_gpu(A, x, dtype, cfg, a.warmup, a.steps, a.flush_mb << 20)
+    vs = np.dtype(dtype).itemsize
+    b = eff_bytes(A.m, A.n, A.nnz, vs)
+    med_total = float(np.median(r["ms_total"]))
+    mean_main = float(np.mean(r["ms_main"]))
+    value = b / (med_total * 1e-3) / 1e9
+    achieved = b / (mean_main * 1e-3) / 1e9
+    prof = f"segsum_{a.workload.replace('-', '')}_{a.dtype}_traffic.json"
+    traffic = load_traffic(prof)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(med_total, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": a.dtype,
+        "data": "synthetic (seeded generator), inputs resident in HBM, L2+MALL flushed per step",
+        "config": {
+            "workload": A.name, "m": A.m, "n": A.n, "nnz": A.nnz,
+            "effective_bytes": b, "kernel": "speculative segmented sum",
+            "items_per_lane": S, "waves_per_block": a.waves_per_block, "tiles_per_wave": tpw,
+            "parallelism": "single GPU",
+        },
+        "pct_hbm_peak": round(100 * value / HBM_PEAK_GBPS, 2),
+        "roofline": {
+            "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": traffic,
+            "kernel": f"segsum_kernel<{'double' if vs == 8 else 'float'},{S}>",
+            "kernel_mean_us": round(mean_main * 1e3, 3),
+        },
+        "timing": {
+            "median_step_ms": med_total, "p10_step_ms": float(np.percentile(r["ms_total"], 10)),
+            "p90_step_ms": float(np.percentile(r["ms_total"], 90)),
+            "mean_main_kernel_ms": mean_main,
+            "wall_s_timed_loop_incl_flush": round(r["wall_s"], 4),
+            "warm_median_step_ms": float(np.median(r["warm_total"])),
+            "warm_gbps": round(b / (float(np.median(r["warm_total"])) * 1e-3) / 1e9, 2),
+            "init_data_s": round(r["init_s"], 4),
+        },
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(A, x, dtype, r["y"])
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

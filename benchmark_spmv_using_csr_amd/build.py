@@ -1,0 +1,75 @@
+"""Build recipes for the native libraries (run by ``__graft_entry__.build()``).
+
+* ``libbhsparse_spmv.so`` -- HIP kernels + the C ABI of include/bhsparse_spmv.h,
+  compiled for gfx950 with hipcc.
+* ``libbhs_host.so``      -- host-side C++ (seeded matrix generators).
+
+Both land next to this file (in-tree, so they travel with the repository).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(PKG, "csrc")
+ROOT = os.path.dirname(PKG)
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+HIP_SOURCES = ["bhs_api.hip"]
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    "-ffp-contract=off",            # products are rounded before they are summed
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+    f"-I{os.path.join(ROOT, 'include')}",
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout)
+        raise RuntimeError(f"build step failed: {cmd[0]} ... -o {cmd[-1] if cmd else ''}")
+    return r.stdout
+
+
+def build_hip(jobs: int = 4) -> str:
+    objs = []
+    cmds = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
+        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(_run, cmds))
+    out = os.path.join(PKG, "libbhsparse_spmv.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_host() -> str:
+    out = os.path.join(PKG, "libbhs_host.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-Wall",
+          os.path.join(CSRC, "matgen.cpp"), "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all() -> None:
+    with ThreadPoolExecutor(max_workers=2) as ex:
+        fh = ex.submit(build_hip)
+        fo = ex.submit(build_host)
+        fh.result()
+        fo.result()
+
+
+if __name__ == "__main__":
+    build_all()
+    print("built", os.path.join(PKG, "libbhsparse_spmv.so"), os.path.join(PKG, "libbhs_host.so"))

@@ -1,0 +1,735 @@
+// bhs_api.hip -- the C ABI of include/bhsparse_spmv.h: device selection,
+// validated upload of A and x, per-matrix tile partition, the spmv launch
+// sequence, timing and debug queries.
+//
+// Reference counterparts: bhsparse_spmv_cuda (spmv_cuda/bhsparse_spmv_cuda.h)
+// and bhsparse_spmv_opencl (spmv_opencl_amd/bhsparse_spmv_opencl.h).  Unlike
+// them, every buffer is plain device memory (hipMalloc), nothing is computed
+// on the host per call, and a call never depends on y's previous contents.
+#include "bhs_kernels.h"
+#include "../../include/bhsparse_spmv.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define BHS_HIP(call)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return set_error(e_ == hipErrorOutOfMemory ? BHSPARSE_OUT_OF_MEMORY          \
+                                                       : BHSPARSE_HIP_ERROR,             \
+                             std::string(#call) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+// The largest tile (64 lanes x 32 items) -- arrays are padded to it.
+constexpr int kMaxTile = 64 * 32;
+// nnz limit keeps every tile index, tile boundary and padded index in int.
+constexpr long long kMaxNnz = 0x7fffffffLL - 4 * kMaxTile;
+
+enum ValueKind { kFloat = 1, kDouble = 2 };
+
+template <typename T>
+struct Impl {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    bool platform = false;
+
+    bool data = false;
+    int m = 0, n = 0, nnz = 0;
+    long long nnz_pad = 0;
+    int *d_row_ptr = nullptr, *d_col = nullptr;
+    T *d_val = nullptr, *d_x = nullptr, *d_y = nullptr;
+    T *h_y = nullptr;   // borrowed, as _h_y in bhsparse_spmv_cuda.h:248
+
+    bhs_config cfg{BHS_VARIANT_SEGSUM, sizeof(T) == 8 ? 8 : 16, 4, sizeof(T) == 8 ? 3 : 2, 0};
+
+    // analysis for the current tile shape
+    int an_P = 0, an_TPW = 0;
+    int NP = 0, NW = 0, lead_empty = 0;
+    int *d_tile_row = nullptr;
+    int tile_cap = 0;
+    int *d_sync_row = nullptr;
+    T *d_sync_val = nullptr;
+    int sync_cap = 0;
+
+    unsigned long long *d_counters = nullptr;   // 8 x u64 scratch for stats/features
+    void *d_flush = nullptr;
+    long long flush_cap = 0;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    size_t max_lds = 65536;   // per-block LDS limit of the device
+};
+
+struct HandleBox {
+    int kind;
+    void *impl;
+};
+
+template <typename T> constexpr int kind_of();
+template <> constexpr int kind_of<float>() { return kFloat; }
+template <> constexpr int kind_of<double>() { return kDouble; }
+
+template <typename T>
+int get_impl(bhs_handle_t h, Impl<T> **out)
+{
+    if (!h) return set_error(BHSPARSE_INVALID_ARGUMENT, "null handle");
+    HandleBox *b = reinterpret_cast<HandleBox *>(h);
+    if (b->kind != kind_of<T>())
+        return set_error(BHSPARSE_UNSUPPORTED_VALUE_TYPE,
+                         "handle was created for the other precision");
+    *out = static_cast<Impl<T> *>(b->impl);
+    return BHSPARSE_SUCCESS;
+}
+
+template <typename T>
+void free_ptr(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+template <typename T>
+int create(bhs_handle_t *h)
+{
+    if (!h) return set_error(BHSPARSE_INVALID_ARGUMENT, "null handle pointer");
+    HandleBox *b = new (std::nothrow) HandleBox;
+    Impl<T> *im = new (std::nothrow) Impl<T>;
+    if (!b || !im) {
+        delete b;
+        delete im;
+        return set_error(BHSPARSE_OUT_OF_MEMORY, "host allocation failed");
+    }
+    b->kind = kind_of<T>();
+    b->impl = im;
+    *h = reinterpret_cast<bhs_handle_t>(b);
+    return BHSPARSE_SUCCESS;
+}
+
+template <typename T>
+int init_platform(bhs_handle_t h, int device)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return set_error(BHSPARSE_UNSUPPORTED_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= count)
+        return set_error(BHSPARSE_UNSUPPORTED_DEVICE, "device index out of range");
+    hipDeviceProp_t prop;
+    BHS_HIP(hipGetDeviceProperties(&prop, device));
+    if (prop.warpSize != 64)
+        return set_error(BHSPARSE_UNSUPPORTED_DEVICE, "wave64 device required");
+    if (im->platform && im->device != device)
+        return set_error(BHSPARSE_INVALID_ARGUMENT, "handle already bound to another device");
+    BHS_HIP(hipSetDevice(device));
+    im->max_lds = std::min<size_t>(prop.sharedMemPerBlock, 65536);
+    if (!im->platform) {
+        BHS_HIP(hipStreamCreateWithFlags(&im->stream, hipStreamNonBlocking));
+        for (auto &e : im->ev) BHS_HIP(hipEventCreate(&e));
+        BHS_HIP(hipMalloc(&im->d_counters, 8 * sizeof(unsigned long long)));
+    }
+    im->device = device;
+    im->platform = true;
+    return BHSPARSE_SUCCESS;
+}
+
+template <typename T>
+void release_data(Impl<T> *im)
+{
+    if (im->platform) (void)hipSetDevice(im->device);
+    if (im->stream) (void)hipStreamSynchronize(im->stream);
+    free_ptr(im->d_row_ptr);
+    free_ptr(im->d_col);
+    free_ptr(im->d_val);
+    free_ptr(im->d_x);
+    free_ptr(im->d_y);
+    free_ptr(im->d_tile_row);
+    free_ptr(im->d_sync_row);
+    free_ptr(im->d_sync_val);
+    im->tile_cap = im->sync_cap = 0;
+    im->an_P = im->an_TPW = 0;
+    im->NP = im->NW = im->lead_empty = 0;
+    im->h_y = nullptr;
+    im->data = false;
+    im->m = im->n = im->nnz = 0;
+}
+
+int check_config(const bhs_config &c)
+{
+    if (c.variant != BHS_VARIANT_SEGSUM && c.variant != BHS_VARIANT_CSRVEC)
+        return set_error(BHSPARSE_INVALID_ARGUMENT, "unknown variant");
+    const int S = c.items_per_lane;
+    if (S != 4 && S != 8 && S != 16 && S != 32)
+        return set_error(BHSPARSE_INVALID_ARGUMENT, "items_per_lane must be 4, 8, 16 or 32");
+    if (c.waves_per_block < 1 || c.waves_per_block > 8)
+        return set_error(BHSPARSE_INVALID_ARGUMENT, "waves_per_block must be in [1, 8]");
+    if (c.tiles_per_wave < 1 || c.tiles_per_wave > 4096)
+        return set_error(BHSPARSE_INVALID_ARGUMENT, "tiles_per_wave must be in [1, 4096]");
+    const int t = c.tpv;
+    if (t != 0 && t != 2 && t != 4 && t != 8 && t != 16 && t != 32 && t != 64)
+        return set_error(BHSPARSE_INVALID_ARGUMENT, "tpv must be 0, 2, 4, 8, 16, 32 or 64");
+    return BHSPARSE_SUCCESS;
+}
+
+// Partition for the configured tile size and sizes of the per-wave buffers.
+template <typename T>
+int ensure_analysis(Impl<T> *im)
+{
+    const int P = 64 * im->cfg.items_per_lane;
+    const int TPW = im->cfg.tiles_per_wave;
+    if (im->an_P == P && im->an_TPW == TPW) return BHSPARSE_SUCCESS;
+    const long long NP = ((long long)im->nnz + P - 1) / P;
+    const long long NW = (NP + TPW - 1) / TPW;
+    if (NP + 1 > im->tile_cap) {
+        free_ptr(im->d_tile_row);
+        BHS_HIP(hipMalloc(&im->d_tile_row, sizeof(int) * (size_t)(NP + 1)));
+        im->tile_cap = (int)(NP + 1);
+    }
+    if (NW > im->sync_cap) {
+        free_ptr(im->d_sync_row);
+        free_ptr(im->d_sync_val);
+        BHS_HIP(hipMalloc(&im->d_sync_row, sizeof(int) * (size_t)std::max<long long>(NW, 1)));
+        BHS_HIP(hipMalloc(&im->d_sync_val, sizeof(T) * (size_t)std::max<long long>(NW, 1)));
+        im->sync_cap = (int)std::max<long long>(NW, 1);
+    }
+    const int threads = 256;
+    const int blocks = (int)((NP + 1 + threads - 1) / threads);
+    hipLaunchKernelGGL(bhs::partition_kernel, dim3(blocks), dim3(threads), 0, im->stream,
+                       im->d_row_ptr, im->m, im->nnz, P, (int)NP, im->d_tile_row);
+    BHS_HIP(hipGetLastError());
+    int first = 0;
+    BHS_HIP(hipMemcpyAsync(&first, im->d_tile_row, sizeof(int), hipMemcpyDeviceToHost, im->stream));
+    BHS_HIP(hipStreamSynchronize(im->stream));
+    im->NP = (int)NP;
+    im->NW = (int)NW;
+    im->lead_empty = first;
+    im->an_P = P;
+    im->an_TPW = TPW;
+    return BHSPARSE_SUCCESS;
+}
+
+template <typename T>
+int init_data(bhs_handle_t h, int m, int n, int nnz, const int *row_ptr, const int *col,
+              const T *val, const T *x, T *y)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (!im->platform) return set_error(BHSPARSE_NOT_INITIALIZED, "init_platform first");
+    if (m < 0 || n < 0 || nnz < 0) return set_error(BHSPARSE_INVALID_ARGUMENT, "negative size");
+    if (!row_ptr) return set_error(BHSPARSE_INVALID_ARGUMENT, "null row_ptr");
+    if (nnz > 0 && (!col || !val)) return set_error(BHSPARSE_INVALID_ARGUMENT, "null col/val");
+    if (n > 0 && !x) return set_error(BHSPARSE_INVALID_ARGUMENT, "null x");
+    if ((long long)nnz > kMaxNnz) return set_error(BHSPARSE_INVALID_MATRIX, "nnz too large");
+    if (row_ptr[0] != 0 || row_ptr[m] != nnz)
+        return set_error(BHSPARSE_INVALID_MATRIX, "row_ptr[0] must be 0 and row_ptr[m] == nnz");
+    for (int i = 0; i < m; i++)
+        if (row_ptr[i + 1] < row_ptr[i])
+            return set_error(BHSPARSE_INVALID_MATRIX, "row_ptr is not monotone");
+    for (int j = 0; j < nnz; j++)
+        if (col[j] < 0 || col[j] >= n)
+            return set_error(BHSPARSE_INVALID_MATRIX, "column index out of range");
+
+    release_data(im);
+    BHS_HIP(hipSetDevice(im->device));
+    im->m = m;
+    im->n = n;
+    im->nnz = nnz;
+    im->nnz_pad = ((long long)nnz + kMaxTile - 1) / kMaxTile * kMaxTile + kMaxTile;
+    BHS_HIP(hipMalloc(&im->d_row_ptr, sizeof(int) * ((size_t)m + 1)));
+    BHS_HIP(hipMalloc(&im->d_col, sizeof(int) * (size_t)im->nnz_pad));
+    BHS_HIP(hipMalloc(&im->d_val, sizeof(T) * (size_t)im->nnz_pad));
+    BHS_HIP(hipMalloc(&im->d_x, sizeof(T) * (size_t)std::max(n, 1)));
+    BHS_HIP(hipMalloc(&im->d_y, sizeof(T) * (size_t)std::max(m, 1)));
+    BHS_HIP(hipMemcpyAsync(im->d_row_ptr, row_ptr, sizeof(int) * ((size_t)m + 1),
+                           hipMemcpyHostToDevice, im->stream));
+    BHS_HIP(hipMemsetAsync(im->d_col, 0, sizeof(int) * (size_t)im->nnz_pad, im->stream));
+    BHS_HIP(hipMemsetAsync(im->d_val, 0, sizeof(T) * (size_t)im->nnz_pad, im->stream));
+    BHS_HIP(hipMemsetAsync(im->d_x, 0, sizeof(T) * (size_t)std::max(n, 1), im->stream));
+    if (nnz > 0) {
+        BHS_HIP(hipMemcpyAsync(im->d_col, col, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice,
+                               im->stream));
+        BHS_HIP(hipMemcpyAsync(im->d_val, val, sizeof(T) * (size_t)nnz, hipMemcpyHostToDevice,
+                               im->stream));
+    }
+    if (n > 0)
+        BHS_HIP(hipMemcpyAsync(im->d_x, x, sizeof(T) * (size_t)n, hipMemcpyHostToDevice,
+                               im->stream));
+    BHS_HIP(hipStreamSynchronize(im->stream));
+    im->h_y = y;
+    im->data = true;
+    return ensure_analysis(im);
+}
+
+template <typename T>
+int set_x(bhs_handle_t h, const T *x)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (!im->data) return set_error(BHSPARSE_NOT_INITIALIZED, "init_data first");
+    if (im->n > 0 && !x) return set_error(BHSPARSE_INVALID_ARGUMENT, "null x");
+    BHS_HIP(hipSetDevice(im->device));
+    if (im->n > 0)
+        BHS_HIP(hipMemcpyAsync(im->d_x, x, sizeof(T) * (size_t)im->n, hipMemcpyHostToDevice,
+                               im->stream));
+    BHS_HIP(hipStreamSynchronize(im->stream));
+    return BHSPARSE_SUCCESS;
+}
+
+template <typename T, int S, bool STATS>
+void launch_segsum_s(Impl<T> *im, T alpha, unsigned long long *stats)
+{
+    bhs::SegsumParams<T> q;
+    q.row_ptr = im->d_row_ptr;
+    q.col = im->d_col;
+    q.val = im->d_val;
+    q.x = im->d_x;
+    q.y = im->d_y;
+    q.tile_row = im->d_tile_row;
+    q.sync_row = im->d_sync_row;
+    q.sync_val = im->d_sync_val;
+    q.stats = stats;
+    q.m = im->m;
+    q.nnz = im->nnz;
+    q.NP = im->NP;
+    q.TPW = im->cfg.tiles_per_wave;
+    q.NW = im->NW;
+    q.lead_empty = im->lead_empty;
+    q.wpb = im->cfg.waves_per_block;
+    q.wave_lds_bytes = (256 + 64 * S * (int)sizeof(T) + 15) / 16 * 16;
+    q.alpha = alpha;
+    const int blocks = (im->NW + q.wpb - 1) / q.wpb;
+    const size_t lds = (size_t)q.wpb * q.wave_lds_bytes;
+    hipLaunchKernelGGL((bhs::segsum_kernel<T, S, STATS>), dim3(blocks), dim3(64 * q.wpb), lds,
+                       im->stream, q);
+}
+
+template <typename T, bool STATS>
+void launch_segsum(Impl<T> *im, T alpha, unsigned long long *stats)
+{
+    switch (im->cfg.items_per_lane) {
+    case 4: launch_segsum_s<T, 4, STATS>(im, alpha, stats); break;
+    case 8: launch_segsum_s<T, 8, STATS>(im, alpha, stats); break;
+    case 16: launch_segsum_s<T, 16, STATS>(im, alpha, stats); break;
+    default: launch_segsum_s<T, 32, STATS>(im, alpha, stats); break;
+    }
+}
+
+template <typename T>
+void launch_calibrate(Impl<T> *im)
+{
+    if (im->NW <= 1) return;   // wave 0 never has a boundary partial
+    const int threads = 256;
+    hipLaunchKernelGGL(bhs::calibrate_kernel<T>, dim3((im->NW + threads - 1) / threads),
+                       dim3(threads), 0, im->stream, im->d_sync_row, im->d_sync_val, im->NW,
+                       im->d_y);
+}
+
+// TPV rule of main.cu:238-287 / csr_vector_opencl.h:212-230 (by floor(nnz/m)).
+inline int tpv_rule(int m, int nnz)
+{
+    const int r = m > 0 ? nnz / m : 0;
+    if (r <= 2) return 2;
+    if (r <= 4) return 4;
+    if (r <= 8) return 8;
+    if (r <= 16) return 16;
+    if (r <= 32) return 32;
+    return 64;
+}
+
+template <typename T, int TPV>
+void launch_csrvec_t(Impl<T> *im, T alpha)
+{
+    const int threads = 256;
+    const long long vpb = threads / TPV;
+    long long blocks = ((long long)im->m + vpb - 1) / vpb;
+    blocks = std::max<long long>(1, std::min<long long>(blocks, 256LL * 32));
+    hipLaunchKernelGGL((bhs::csrvec_kernel<T, TPV>), dim3((unsigned)blocks), dim3(threads), 0,
+                       im->stream, im->m, im->d_row_ptr, im->d_col, im->d_val, im->d_x, im->d_y,
+                       alpha);
+}
+
+template <typename T>
+void launch_csrvec(Impl<T> *im, T alpha)
+{
+    const int tpv = im->cfg.tpv ? im->cfg.tpv : tpv_rule(im->m, im->nnz);
+    switch (tpv) {
+    case 2: launch_csrvec_t<T, 2>(im, alpha); break;
+    case 4: launch_csrvec_t<T, 4>(im, alpha); break;
+    case 8: launch_csrvec_t<T, 8>(im, alpha); break;
+    case 16: launch_csrvec_t<T, 16>(im, alpha); break;
+    case 32: launch_csrvec_t<T, 32>(im, alpha); break;
+    default: launch_csrvec_t<T, 64>(im, alpha); break;
+    }
+}
+
+template <typename T, typename U>
+void launch_fill(Impl<T> *im, U *p, long long n)
+{
+    if (n <= 0) return;
+    const int threads = 256;
+    const long long blocks = std::min<long long>((n + threads - 1) / threads, 4096);
+    hipLaunchKernelGGL(bhs::fill_kernel<U>, dim3((unsigned)blocks), dim3(threads), 0, im->stream,
+                       p, n, U(0));
+}
+
+template <typename T>
+size_t segsum_lds_bytes(const bhs_config &c)
+{
+    const size_t per_wave = (256 + 64 * (size_t)c.items_per_lane * sizeof(T) + 15) / 16 * 16;
+    return per_wave * (size_t)c.waves_per_block;
+}
+
+// Enqueues one y = alpha*A*x.  `main_done` (optional) is recorded right after
+// the main kernel.
+template <typename T>
+int enqueue_spmv(Impl<T> *im, T alpha, hipEvent_t main_done = nullptr)
+{
+    if (!im->data) return set_error(BHSPARSE_NOT_INITIALIZED, "init_data first");
+    if (int rc = ensure_analysis(im)) return rc;
+    if (im->m == 0) {
+        if (main_done) BHS_HIP(hipEventRecord(main_done, im->stream));
+        return BHSPARSE_SUCCESS;
+    }
+    if (im->cfg.variant == BHS_VARIANT_CSRVEC) {
+        launch_csrvec(im, alpha);
+        BHS_HIP(hipGetLastError());
+        if (main_done) BHS_HIP(hipEventRecord(main_done, im->stream));
+        return BHSPARSE_SUCCESS;
+    }
+    if (im->NP == 0) {                        // no nonzeros: y = 0
+        launch_fill(im, im->d_y, im->m);
+        BHS_HIP(hipGetLastError());
+        if (main_done) BHS_HIP(hipEventRecord(main_done, im->stream));
+        return BHSPARSE_SUCCESS;
+    }
+    if (segsum_lds_bytes<T>(im->cfg) > im->max_lds)
+        return set_error(BHSPARSE_INVALID_ARGUMENT,
+                         "items_per_lane x waves_per_block needs more LDS than a block may use");
+    launch_segsum<T, false>(im, alpha, nullptr);
+    BHS_HIP(hipGetLastError());
+    if (main_done) BHS_HIP(hipEventRecord(main_done, im->stream));
+    launch_calibrate(im);
+    BHS_HIP(hipGetLastError());
+    return BHSPARSE_SUCCESS;
+}
+
+template <typename T>
+int spmv_async(bhs_handle_t h, T alpha)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (!im->platform) return set_error(BHSPARSE_NOT_INITIALIZED, "init_platform first");
+    BHS_HIP(hipSetDevice(im->device));
+    return enqueue_spmv(im, alpha);
+}
+
+template <typename T>
+int spmv(bhs_handle_t h, T alpha)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (int rc = spmv_async<T>(h, alpha)) return rc;
+    BHS_HIP(hipStreamSynchronize(im->stream));
+    return BHSPARSE_SUCCESS;
+}
+
+template <typename T>
+int get_y_to(bhs_handle_t h, T *out)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (!im->data) return set_error(BHSPARSE_NOT_INITIALIZED, "init_data first");
+    if (im->m > 0 && !out) return set_error(BHSPARSE_INVALID_ARGUMENT, "null y");
+    BHS_HIP(hipSetDevice(im->device));
+    if (im->m > 0)
+        BHS_HIP(hipMemcpyAsync(out, im->d_y, sizeof(T) * (size_t)im->m, hipMemcpyDeviceToHost,
+                               im->stream));
+    BHS_HIP(hipStreamSynchronize(im->stream));
+    return BHSPARSE_SUCCESS;
+}
+
+template <typename T>
+int get_y(bhs_handle_t h)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    return get_y_to<T>(h, im->h_y);
+}
+
+template <typename T>
+int free_mem(bhs_handle_t h)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    release_data(im);
+    if (im->d_flush) {
+        (void)hipFree(im->d_flush);
+        im->d_flush = nullptr;
+        im->flush_cap = 0;
+    }
+    return BHSPARSE_SUCCESS;
+}
+
+template <typename T>
+int free_platform(bhs_handle_t h)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (!im->platform) return BHSPARSE_SUCCESS;
+    release_data(im);
+    (void)hipSetDevice(im->device);
+    if (im->d_flush) (void)hipFree(im->d_flush);
+    im->d_flush = nullptr;
+    im->flush_cap = 0;
+    free_ptr(im->d_counters);
+    for (auto &e : im->ev)
+        if (e) { (void)hipEventDestroy(e); e = nullptr; }
+    if (im->stream) (void)hipStreamDestroy(im->stream);
+    im->stream = nullptr;
+    im->platform = false;
+    return BHSPARSE_SUCCESS;
+}
+
+template <typename T>
+int destroy(bhs_handle_t h)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    free_platform<T>(h);
+    delete im;
+    delete reinterpret_cast<HandleBox *>(h);
+    return BHSPARSE_SUCCESS;
+}
+
+template <typename T>
+int set_config(bhs_handle_t h, const bhs_config *c)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (!c) return set_error(BHSPARSE_INVALID_ARGUMENT, "null config");
+    if (int rc = check_config(*c)) return rc;
+    im->cfg = *c;
+    if (im->data) {
+        BHS_HIP(hipSetDevice(im->device));
+        return ensure_analysis(im);
+    }
+    return BHSPARSE_SUCCESS;
+}
+
+template <typename T>
+int get_config(bhs_handle_t h, bhs_config *c)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (!c) return set_error(BHSPARSE_INVALID_ARGUMENT, "null config");
+    *c = im->cfg;
+    return BHSPARSE_SUCCESS;
+}
+
+template <typename T>
+int get_partition(bhs_handle_t h, int *tile_row, int cap, int *np)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (!im->data) return set_error(BHSPARSE_NOT_INITIALIZED, "init_data first");
+    BHS_HIP(hipSetDevice(im->device));
+    if (int rc = ensure_analysis(im)) return rc;
+    if (np) *np = im->NP;
+    if (tile_row) {
+        if (cap < im->NP + 1) return set_error(BHSPARSE_INVALID_ARGUMENT, "capacity < NP+1");
+        BHS_HIP(hipMemcpyAsync(tile_row, im->d_tile_row, sizeof(int) * ((size_t)im->NP + 1),
+                               hipMemcpyDeviceToHost, im->stream));
+        BHS_HIP(hipStreamSynchronize(im->stream));
+    }
+    return BHSPARSE_SUCCESS;
+}
+
+template <typename T>
+int get_stats(bhs_handle_t h, bhs_spmv_stats *out)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (!out) return set_error(BHSPARSE_INVALID_ARGUMENT, "null stats");
+    if (!im->data) return set_error(BHSPARSE_NOT_INITIALIZED, "init_data first");
+    BHS_HIP(hipSetDevice(im->device));
+    if (int rc = ensure_analysis(im)) return rc;
+    std::memset(out, 0, sizeof(*out));
+    out->n_tiles = im->NP;
+    out->n_waves = im->NW;
+    out->n_lead_empty = im->lead_empty;
+    if (im->NP == 0 || im->m == 0) return BHSPARSE_SUCCESS;
+    if (segsum_lds_bytes<T>(im->cfg) > im->max_lds)
+        return set_error(BHSPARSE_INVALID_ARGUMENT,
+                         "items_per_lane x waves_per_block needs more LDS than a block may use");
+    launch_fill(im, im->d_counters, 8);
+    launch_segsum<T, true>(im, T(1), im->d_counters);
+    BHS_HIP(hipGetLastError());
+    launch_calibrate(im);
+    BHS_HIP(hipGetLastError());
+    unsigned long long c[3];
+    BHS_HIP(hipMemcpyAsync(c, im->d_counters, sizeof(c), hipMemcpyDeviceToHost, im->stream));
+    BHS_HIP(hipStreamSynchronize(im->stream));
+    out->n_fast = (long long)c[0];
+    out->n_dirty = (long long)c[1];
+    out->n_sync = (long long)c[2];
+    return BHSPARSE_SUCCESS;
+}
+
+template <typename T>
+int bench(bhs_handle_t h, T alpha, int warmup, int reps, long long flush_bytes, double *ms_total,
+          double *ms_main)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (!im->data) return set_error(BHSPARSE_NOT_INITIALIZED, "init_data first");
+    if (warmup < 0 || reps < 0 || flush_bytes < 0)
+        return set_error(BHSPARSE_INVALID_ARGUMENT, "negative count");
+    BHS_HIP(hipSetDevice(im->device));
+    const long long n16 = flush_bytes / 16;
+    if (n16 > 0 && flush_bytes > im->flush_cap) {
+        if (im->d_flush) (void)hipFree(im->d_flush);
+        im->d_flush = nullptr;
+        im->flush_cap = 0;
+        BHS_HIP(hipMalloc(&im->d_flush, (size_t)n16 * 16));
+        im->flush_cap = n16 * 16;
+    }
+    auto flush = [&]() {
+        if (n16 <= 0) return;
+        hipLaunchKernelGGL(bhs::flush_kernel, dim3(4096), dim3(256), 0, im->stream,
+                           reinterpret_cast<bhs::u32x4 *>(im->d_flush), n16, 0u);
+    };
+    for (int i = 0; i < warmup; ++i) {
+        flush();
+        if (int rc = enqueue_spmv(im, alpha)) return rc;
+    }
+    BHS_HIP(hipStreamSynchronize(im->stream));
+    for (int i = 0; i < reps; ++i) {
+        flush();
+        BHS_HIP(hipEventRecord(im->ev[0], im->stream));
+        if (int rc = enqueue_spmv(im, alpha, im->ev[1])) return rc;
+        BHS_HIP(hipEventRecord(im->ev[2], im->stream));
+        BHS_HIP(hipEventSynchronize(im->ev[2]));
+        float a = 0, b = 0;
+        BHS_HIP(hipEventElapsedTime(&a, im->ev[0], im->ev[2]));
+        BHS_HIP(hipEventElapsedTime(&b, im->ev[0], im->ev[1]));
+        if (ms_total) ms_total[i] = a;
+        if (ms_main) ms_main[i] = b;
+    }
+    return BHSPARSE_SUCCESS;
+}
+
+template <typename T>
+int row_features(bhs_handle_t h, double *f)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (!f) return set_error(BHSPARSE_INVALID_ARGUMENT, "null output");
+    if (!im->data) return set_error(BHSPARSE_NOT_INITIALIZED, "init_data first");
+    BHS_HIP(hipSetDevice(im->device));
+    launch_fill(im, im->d_counters, 8);
+    if (im->m > 0) {
+        const int threads = 256;
+        const int blocks = (int)std::min<long long>(((long long)im->m + threads - 1) / threads, 2048);
+        hipLaunchKernelGGL(bhs::row_stats_kernel, dim3(blocks), dim3(threads), 0, im->stream,
+                           im->m, im->d_row_ptr, im->d_col, im->d_counters);
+        BHS_HIP(hipGetLastError());
+    }
+    unsigned long long c[5];
+    BHS_HIP(hipMemcpyAsync(c, im->d_counters, sizeof(c), hipMemcpyDeviceToHost, im->stream));
+    BHS_HIP(hipStreamSynchronize(im->stream));
+    const double m = im->m, nnz = im->nnz;
+    const double mean = m > 0 ? (double)c[0] / m : 0.0;
+    const double var = m > 0 ? (double)c[1] / m - mean * mean : 0.0;
+    const double nonempty = m - (double)c[2];
+    f[0] = m;
+    f[1] = nnz;
+    f[2] = mean;
+    f[3] = var > 0 ? var : 0.0;
+    f[4] = (double)c[4];
+    f[5] = m > 0 ? (double)c[2] / m : 0.0;
+    f[6] = nonempty > 0 ? (double)c[3] / nonempty : 0.0;
+    f[7] = sizeof(T) == 8 ? 1.0 : 0.0;
+    return BHSPARSE_SUCCESS;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *bhs_error_string(int code)
+{
+    switch (code) {
+    case BHSPARSE_SUCCESS: return "success";
+    case BHSPARSE_UNSUPPORTED_VALUE_TYPE: return "unsupported value type";
+    case BHSPARSE_UNSUPPORTED_DEVICE: return "unsupported device";
+    case BHSPARSE_INVALID_ARGUMENT: return "invalid argument";
+    case BHSPARSE_INVALID_MATRIX: return "invalid matrix";
+    case BHSPARSE_OUT_OF_MEMORY: return "out of device memory";
+    case BHSPARSE_HIP_ERROR: return "HIP runtime error";
+    case BHSPARSE_NOT_INITIALIZED: return "not initialized";
+    default: return "unknown error";
+    }
+}
+
+const char *bhs_last_error(void) { return g_last_error.c_str(); }
+
+int bhs_device_synchronize(void)
+{
+    BHS_HIP(hipDeviceSynchronize());
+    return BHSPARSE_SUCCESS;
+}
+
+int bhs_device_count(void)
+{
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+#define BHS_DEFINE_API(P, T)                                                                   \
+    int bhs_##P##_create(bhs_handle_t *h) { return create<T>(h); }                            \
+    int bhs_##P##_init_platform(bhs_handle_t h, int d) { return init_platform<T>(h, d); }    \
+    int bhs_##P##_init_data(bhs_handle_t h, int m, int n, int nnz, const int *rp,             \
+                            const int *ci, const T *v, const T *x, T *y)                      \
+    {                                                                                          \
+        return init_data<T>(h, m, n, nnz, rp, ci, v, x, y);                                    \
+    }                                                                                          \
+    int bhs_##P##_set_x(bhs_handle_t h, const T *x) { return set_x<T>(h, x); }                \
+    int bhs_##P##_spmv(bhs_handle_t h, T a) { return spmv<T>(h, a); }                         \
+    int bhs_##P##_spmv_async(bhs_handle_t h, T a) { return spmv_async<T>(h, a); }             \
+    int bhs_##P##_get_y(bhs_handle_t h) { return get_y<T>(h); }                               \
+    int bhs_##P##_get_y_to(bhs_handle_t h, T *o) { return get_y_to<T>(h, o); }                \
+    int bhs_##P##_free_mem(bhs_handle_t h) { return free_mem<T>(h); }                         \
+    int bhs_##P##_free_platform(bhs_handle_t h) { return free_platform<T>(h); }               \
+    int bhs_##P##_destroy(bhs_handle_t h) { return destroy<T>(h); }                           \
+    int bhs_##P##_set_config(bhs_handle_t h, const bhs_config *c) { return set_config<T>(h, c); } \
+    int bhs_##P##_get_config(bhs_handle_t h, bhs_config *c) { return get_config<T>(h, c); }   \
+    int bhs_##P##_get_partition(bhs_handle_t h, int *t, int cap, int *np)                     \
+    {                                                                                          \
+        return get_partition<T>(h, t, cap, np);                                                \
+    }                                                                                          \
+    int bhs_##P##_get_stats(bhs_handle_t h, bhs_spmv_stats *o) { return get_stats<T>(h, o); } \
+    int bhs_##P##_bench(bhs_handle_t h, T a, int w, int r, long long fb, double *mt,          \
+                        double *mm)                                                            \
+    {                                                                                          \
+        return bench<T>(h, a, w, r, fb, mt, mm);                                               \
+    }                                                                                          \
+    int bhs_##P##_row_features(bhs_handle_t h, double *f) { return row_features<T>(h, f); }
+
+BHS_DEFINE_API(s, float)
+BHS_DEFINE_API(d, double)
+
+} // extern "C"

@@ -1,0 +1,147 @@
+/*
+ * bhsparse_spmv.h -- C ABI of the MI355X (gfx950) speculative segmented-sum
+ * CSR SpMV, y = alpha * A * x.
+ *
+ * Drop-in for the operator class of weifengliu-ssslab/Benchmark_SpMV_using_CSR:
+ *   class bhsparse_spmv_cuda        spmv_cuda/bhsparse_spmv_cuda.h:7-55
+ *   class bhsparse_spmv_opencl      spmv_opencl_amd/bhsparse_spmv_opencl.h:7-19
+ * Every entry point exists twice: bhs_s_* computes in float (the reference's
+ * USE_DOUBLE=0 build, spmv_cuda/common.h:27-31) and bhs_d_* in double
+ * (USE_DOUBLE=1).  Plain pointers and sizes only; all indices are 32-bit
+ * (0-based CSR, as in the reference).  Functions return BHSPARSE_SUCCESS (0)
+ * or a negative code and never exit the process.
+ *
+ * A header-only C++ class with the reference's method names sits in
+ * bhsparse_spmv.hpp.
+ */
+#ifndef BHSPARSE_SPMV_H
+#define BHSPARSE_SPMV_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes.  0, -1 and -2 keep the reference's meaning
+ * (spmv_cuda/common.h:24-25, spmv_opencl_amd/common.h:20-21). */
+#define BHSPARSE_SUCCESS                 0
+#define BHSPARSE_UNSUPPORTED_VALUE_TYPE -1
+#define BHSPARSE_UNSUPPORTED_DEVICE     -2
+#define BHSPARSE_INVALID_ARGUMENT       -3   /* null pointer, negative size, bad config */
+#define BHSPARSE_INVALID_MATRIX         -4   /* row_ptr not monotone, col out of range, nnz too large */
+#define BHSPARSE_OUT_OF_MEMORY          -5
+#define BHSPARSE_HIP_ERROR              -6   /* a HIP runtime call failed; see bhs_last_error() */
+#define BHSPARSE_NOT_INITIALIZED        -7   /* call order violated (e.g. spmv before initData) */
+
+/* Kernel variants a handle can run (bhs_*_set_config). */
+#define BHS_VARIANT_SEGSUM  0   /* speculative segmented sum (kernels_cuda.h:7-269) */
+#define BHS_VARIANT_CSRVEC  1   /* csr-vector (kernels_cuda.h:272-342, CUSP v0.4.0) */
+
+typedef struct bhs_handle_opaque *bhs_handle_t;
+
+/* Counters of one segmented-sum launch (tiles of P = 64*S nonzeros, T tiles
+ * per wave).  Filled by bhs_*_get_stats. */
+typedef struct {
+    long long n_tiles;       /* NP = ceil(nnz / P) */
+    long long n_waves;       /* ceil(NP / T) */
+    long long n_fast;        /* tiles inside one row (fast track) */
+    long long n_dirty;       /* tiles holding an empty row (speculation fixed up) */
+    long long n_sync;        /* waves that sent a boundary-row partial to calibration */
+    long long n_lead_empty;  /* empty rows before the first nonzero */
+} bhs_spmv_stats;
+
+/* Tile shape / variant.  S = items per lane (4, 8, 16, 32), waves_per_block
+ * in [1, 8], tiles_per_wave >= 1; tpv (csr-vector lanes per row) in
+ * {0 = rule of main.cu:238-287 extended to 64, 2, 4, 8, 16, 32, 64}. */
+typedef struct {
+    int variant;
+    int items_per_lane;
+    int waves_per_block;
+    int tiles_per_wave;
+    int tpv;
+} bhs_config;
+
+/* ---- library-wide ------------------------------------------------------ */
+const char *bhs_error_string(int code);
+/* Text of the last error raised on this thread (HIP message included). */
+const char *bhs_last_error(void);
+/* hipDeviceSynchronize on the current device. */
+int bhs_device_synchronize(void);
+/* Number of visible devices (0 when none). */
+int bhs_device_count(void);
+
+/* ---- fp32 build (value_type = float) ---------------------------------- */
+/* new bhsparse_spmv_cuda() -- main.cu:772 */
+int bhs_s_create(bhs_handle_t *h);
+/* init_platform(): select the device, create the handle's stream
+ * (bhsparse_spmv_cuda.h:61-75; the reference always uses device 0). */
+int bhs_s_init_platform(bhs_handle_t h, int device);
+/* prepare_mem(m, n, nnzA, csrRowPtrA, csrColIdxA, csrValA, x, y)
+ * (bhsparse_spmv_cuda.h:238-351): validates and copies A and x to the device
+ * and partitions the nonzeros into tiles.  y is borrowed, as in the reference
+ * (:248): get_y() writes m values into it later.  Its contents are never read
+ * (beta = 0). */
+int bhs_s_init_data(bhs_handle_t h, int m, int n, int nnz, const int *row_ptr,
+                    const int *col, const float *val, const float *x, float *y);
+/* Replace x (n values) without re-uploading A. No reference counterpart
+ * (the reference re-runs prepare_mem). */
+int bhs_s_set_x(bhs_handle_t h, const float *x);
+/* run_benchmark() (bhsparse_spmv_cuda.h:121-228), plus alpha, which the
+ * reference drops (utils_cuda.h:133).  Blocking, like the reference. */
+int bhs_s_spmv(bhs_handle_t h, float alpha);
+/* The same work enqueued on the handle's stream without waiting. */
+int bhs_s_spmv_async(bhs_handle_t h, float alpha);
+/* get_y() (bhsparse_spmv_cuda.h:230-236): copies y into the pointer given to
+ * init_data. */
+int bhs_s_get_y(bhs_handle_t h);
+/* Copies y into `out` (m values) instead of the borrowed pointer. */
+int bhs_s_get_y_to(bhs_handle_t h, float *out);
+/* free_mem() (bhsparse_spmv_cuda.h:84-118); safe to call twice. */
+int bhs_s_free_mem(bhs_handle_t h);
+/* free_platform() (bhsparse_spmv_cuda.h:77-82). */
+int bhs_s_free_platform(bhs_handle_t h);
+/* delete: frees everything the handle still owns. */
+int bhs_s_destroy(bhs_handle_t h);
+/* Tile shape and variant (compile-time THREADBUNCH/SEG_H/STEP in the
+ * reference, spmv_cuda/common.h:39-48).  Re-partitions if the tile size changes. */
+int bhs_s_set_config(bhs_handle_t h, const bhs_config *cfg);
+int bhs_s_get_config(bhs_handle_t h, bhs_config *cfg);
+/* Debug: copies tile_row[0..NP] (first row of every tile boundary) into
+ * tile_row (capacity cap) and NP into *np. */
+int bhs_s_get_partition(bhs_handle_t h, int *tile_row, int cap, int *np);
+/* Debug: runs one counting launch of the segmented sum and reports its counters. */
+int bhs_s_get_stats(bhs_handle_t h, bhs_spmv_stats *out);
+/* Timing: `warmup` untimed calls, then `reps` calls each bracketed by HIP
+ * events on the handle's stream.  If flush_bytes > 0, a buffer of that size is
+ * written before every call (outside the events) so caches start cold.
+ * ms_total[i] = whole call, ms_main[i] = main kernel only (both may be NULL). */
+int bhs_s_bench(bhs_handle_t h, float alpha, int warmup, int reps, long long flush_bytes,
+                double *ms_total, double *ms_main);
+/* GPU reduction over row_ptr/col: 8 row-length features (m, nnz, mean, var,
+ * max, empty fraction, bandwidth estimate, precision flag) used by the tuner. */
+int bhs_s_row_features(bhs_handle_t h, double *feat8);
+
+/* ---- fp64 build (value_type = double) --------------------------------- */
+int bhs_d_create(bhs_handle_t *h);
+int bhs_d_init_platform(bhs_handle_t h, int device);
+int bhs_d_init_data(bhs_handle_t h, int m, int n, int nnz, const int *row_ptr,
+                    const int *col, const double *val, const double *x, double *y);
+int bhs_d_set_x(bhs_handle_t h, const double *x);
+int bhs_d_spmv(bhs_handle_t h, double alpha);
+int bhs_d_spmv_async(bhs_handle_t h, double alpha);
+int bhs_d_get_y(bhs_handle_t h);
+int bhs_d_get_y_to(bhs_handle_t h, double *out);
+int bhs_d_free_mem(bhs_handle_t h);
+int bhs_d_free_platform(bhs_handle_t h);
+int bhs_d_destroy(bhs_handle_t h);
+int bhs_d_set_config(bhs_handle_t h, const bhs_config *cfg);
+int bhs_d_get_config(bhs_handle_t h, bhs_config *cfg);
+int bhs_d_get_partition(bhs_handle_t h, int *tile_row, int cap, int *np);
+int bhs_d_get_stats(bhs_handle_t h, bhs_spmv_stats *out);
+int bhs_d_bench(bhs_handle_t h, double alpha, int warmup, int reps, long long flush_bytes,
+                double *ms_total, double *ms_main);
+int bhs_d_row_features(bhs_handle_t h, double *feat8);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BHSPARSE_SPMV_H */

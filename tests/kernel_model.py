@@ -1,0 +1,204 @@
+"""Lane-exact CPU model of the HIP segmented-sum kernel (test helper).
+
+Mirrors benchmark_spmv_using_csr_amd/csrc/bhs_kernels.h::segsum_kernel and
+calibrate_kernel operation by operation, in the kernel's own precision and
+summation order (products rounded before they are summed; the library is built
+with -ffp-contract=off), so the GPU result must equal this model bit for bit.
+The model itself is checked against the FP64 oracle in test_model.py; it is a
+description of the kernel, not the parity oracle.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+W = 64
+FULL = (1 << 64) - 1
+
+
+def partition(row_ptr, nnz, P):
+    NP = (nnz + P - 1) // P
+    keys = np.minimum(np.arange(NP + 1, dtype=np.int64) * P, nnz)
+    return (np.searchsorted(row_ptr, keys, side="right") - 1).astype(np.int32)
+
+
+def _wave_sum(v):
+    v = v.copy()
+    lanes = np.arange(W)
+    off = W // 2
+    while off > 0:
+        v = v + v[lanes ^ off]
+        off >>= 1
+    return v
+
+
+def _segscan(v, present):
+    """Hillis-Steele segmented inclusive scan with the kernel's early exit."""
+    cov = 0
+    for l in range(W):
+        if present[l]:
+            cov |= 1 << l
+    v = v.copy()
+    d = 1
+    while d < W:
+        if cov == FULL:
+            break
+        up = v.copy()
+        up[d:] = v[:-d]              # __shfl_up: lanes below d keep their own value
+        for l in range(W):
+            if not (cov >> l) & 1:
+                v[l] = up[l] + v[l]
+        cov = (cov | (cov << d)) & FULL
+        d <<= 1
+    return v
+
+
+def segsum(row_ptr, col, val, x, S=8, TPW=3, alpha=1.0, dtype=np.float64, counters=None):
+    """y = alpha*A*x exactly as segsum_kernel + calibrate_kernel compute it."""
+    T = np.dtype(dtype).type
+    row_ptr = np.asarray(row_ptr, np.int64)
+    m = len(row_ptr) - 1
+    nnz = int(row_ptr[-1]) if m >= 0 else 0
+    P = W * S
+    NP = (nnz + P - 1) // P
+    NW = (NP + TPW - 1) // TPW
+    y = np.full(max(m, 0), np.nan, dtype)
+    alpha = T(alpha)
+    st = {"n_fast": 0, "n_dirty": 0, "n_sync": 0}
+    if counters is not None:
+        counters.update(st)
+    if m == 0:
+        return y
+    if NP == 0:
+        y[:] = 0
+        return y
+    tile_row = partition(row_ptr, nnz, P)
+    lead_empty = int(tile_row[0])
+    pad = NP * P
+    colp = np.zeros(pad, np.int64)
+    colp[:nnz] = col
+    valp = np.zeros(pad, dtype)
+    valp[:nnz] = np.asarray(val, dtype)
+    xx = np.asarray(x, dtype)
+    if len(xx) == 0:
+        xx = np.zeros(1, dtype)
+    idx = np.arange(pad)
+    prod_all = np.where(idx < nnz, valp * xx[colp], T(0)).astype(dtype)
+
+    sync_row = np.full(NW, -2, np.int64)
+    sync_val = np.zeros(NW, dtype)
+    lanes = np.arange(W)
+    for w in range(NW):
+        if lead_empty > 0:
+            lo = lead_empty * w // NW
+            hi = lead_empty * (w + 1) // NW
+            y[lo:hi] = 0
+        p_begin = w * TPW
+        p_end = min(p_begin + TPW, NP)
+        E0 = p_begin * P
+        r_first = int(tile_row[p_begin])
+        syncing = row_ptr[r_first] < E0
+        did_sync = False
+        is_open = False
+        carry = T(0)
+        for p in range(p_begin, p_end):
+            B = p * P
+            r0 = int(tile_row[p])
+            r1 = int(tile_row[p + 1])
+            cont = row_ptr[r0] < B
+            prod = prod_all[B:B + P].reshape(W, S).copy()
+            if r0 == r1:
+                s = prod[:, 0].copy()
+                for k in range(1, S):
+                    s = s + prod[:, k]
+                s = _wave_sum(s)
+                carry = T(carry + s[0]) if cont else T(s[0])
+                is_open = True
+                st["n_fast"] += 1
+                continue
+            stop = m - 1 if r1 == m else r1
+            hb = np.zeros(W, np.int64)
+            dirty = False
+            for r in range(r0, stop + 1):
+                if row_ptr[r + 1] > row_ptr[r]:
+                    o = int(row_ptr[r] - B)
+                    if 0 <= o < P:
+                        hb[o // S] |= 1 << (o % S)
+                else:
+                    dirty = True
+                    y[r] = 0
+            hb[0] |= 1
+            if cont:
+                prod[0, 0] = T(prod[0, 0] + carry)
+            nh = np.array([bin(int(h)).count("1") for h in hb])
+            incl = np.cumsum(nh)
+            base = incl - nh
+            K = int(incl[-1])
+            s_seg = np.full(P, np.nan, dtype)
+            acc = np.zeros(W, dtype)
+            lead = np.zeros(W, dtype)
+            seen = np.zeros(W, bool)
+            sidx = base.copy()
+            for k in range(S):
+                h = ((hb >> k) & 1).astype(bool)
+                for l in lanes[h]:
+                    if seen[l]:
+                        s_seg[sidx[l]] = acc[l]
+                        sidx[l] += 1
+                    else:
+                        lead[l] = acc[l]
+                        seen[l] = True
+                acc[h] = 0
+                acc = acc + prod[:, k]
+            I = _segscan(acc, seen)
+            for l in range(1, W):
+                if seen[l]:
+                    s_seg[base[l] - 1] = I[l - 1] + lead[l]
+            s_seg[K - 1] = I[W - 1]
+            partial_tail = r1 < m and row_ptr[r1] < B + P
+            kb = 1 if syncing else 0
+            ke = K - 1 if partial_tail else K
+            if not dirty:
+                for k in range(kb, ke):
+                    y[r0 + k] = alpha * s_seg[k]
+            else:
+                st["n_dirty"] += 1
+                running = 0
+                for r in range(r0, stop + 1):
+                    if row_ptr[r + 1] > row_ptr[r]:
+                        if kb <= running < ke:
+                            y[r] = alpha * s_seg[running]
+                        running += 1
+            if syncing:
+                sync_row[w] = r0
+                sync_val[w] = alpha * s_seg[0]
+                syncing = False
+                did_sync = True
+            carry = T(s_seg[K - 1])
+            is_open = partial_tail
+        if is_open:
+            r_open = int(tile_row[p_end])
+            if syncing:
+                sync_row[w] = r_open
+                sync_val[w] = alpha * carry
+                did_sync = True
+            else:
+                y[r_open] = alpha * carry
+        if not did_sync:
+            sync_row[w] = -1
+        if did_sync:
+            st["n_sync"] += 1
+    for w in range(NW):
+        r = sync_row[w]
+        if r < 0:
+            continue
+        if w > 0 and sync_row[w - 1] == r:
+            continue
+        a = T(y[r])
+        i = w
+        while i < NW and sync_row[i] == r:
+            a = T(a + sync_val[i])
+            i += 1
+        y[r] = a
+    if counters is not None:
+        counters.update(st)
+    return y

@@ -1,0 +1,58 @@
+"""The C ABI library loads, exports every symbol of include/bhsparse_spmv.h,
+and its host-side argument checks behave -- without touching a GPU."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from benchmark_spmv_using_csr_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "bhsparse_spmv.h")).read()
+    return sorted(set(re.findall(r"\b(bhs_\w+)\s*\(", txt)))
+
+
+def test_header_and_binding_agree():
+    assert _declared() == sorted(_lib.exported_names())
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.spmv_lib()
+    for name in _declared():
+        assert hasattr(L, name), name
+
+
+def test_error_strings():
+    L = _lib.spmv_lib()
+    assert L.bhs_error_string(0) == b"success"
+    assert L.bhs_error_string(-4) == b"invalid matrix"
+    assert L.bhs_error_string(-99) == b"unknown error"
+
+
+def test_host_side_argument_checks():
+    L = _lib.spmv_lib()
+    h = C.c_void_p()
+    assert L.bhs_d_create(C.byref(h)) == 0
+    # wrong precision for the handle
+    assert L.bhs_s_spmv(h, C.c_float(1.0)) == _lib.BHSPARSE_UNSUPPORTED_VALUE_TYPE
+    # calls before init_platform / init_data
+    rp = np.array([0, 1], np.int32)
+    ci = np.array([0], np.int32)
+    v = np.ones(1)
+    assert L.bhs_d_init_data(h, 1, 1, 1, rp, ci, v, v, None) == _lib.BHSPARSE_NOT_INITIALIZED
+    assert L.bhs_d_spmv(h, 1.0) == _lib.BHSPARSE_NOT_INITIALIZED
+    bad = _lib.Config(7, 8, 4, 3, 0)
+    assert L.bhs_d_set_config(h, C.byref(bad)) == _lib.BHSPARSE_INVALID_ARGUMENT
+    bad = _lib.Config(0, 6, 4, 3, 0)
+    assert L.bhs_d_set_config(h, C.byref(bad)) == _lib.BHSPARSE_INVALID_ARGUMENT
+    good = _lib.Config(0, 16, 2, 5, 0)
+    assert L.bhs_d_set_config(h, C.byref(good)) == 0
+    out = _lib.Config()
+    assert L.bhs_d_get_config(h, C.byref(out)) == 0 and out.as_tuple() == (0, 16, 2, 5, 0)
+    assert L.bhs_d_free_mem(h) == 0 and L.bhs_d_free_mem(h) == 0     # twice is fine
+    assert L.bhs_d_destroy(h) == 0
+    assert L.bhs_d_spmv(None, 1.0) == _lib.BHSPARSE_INVALID_ARGUMENT
