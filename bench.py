@@ -26,7 +26,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from benchmark_spmv_using_csr_amd import _lib, matgen  # noqa: E402
-from benchmark_spmv_using_csr_amd.spmv import bhsparse_spmv, device_synchronize  # noqa: E402
+from benchmark_spmv_using_csr_amd.spmv import (bhsparse_spmv, device_synchronize,  # noqa: E402
+                                               stream_copy)
 
 HBM_PEAK_GBPS = 8000.0
 METRIC = "effective GB/s (% HBM peak) CSR SpMV; tuned+batched vs plain speedup at equal err"
@@ -66,12 +67,13 @@ def run_gpu(A, x, dtype, cfg, warmup, steps, flush_bytes):
     device_synchronize()
     wall = time.perf_counter() - w0
     _lib.check(op.get_y(), "get_y")
-    # warm numbers (no flush) for reference
+    # warm numbers (no flush) and the matrix's data-movement floor
     wt, wm = op.bench(1.0, warmup=3, reps=max(10, steps // 2), flush_bytes=0)
+    fl = op.bench_floor(reps=max(10, steps // 2), flush_bytes=flush_bytes)
     op.close()
     return {
         "ms_total": ms_total, "ms_main": ms_main, "wall_s": wall, "init_s": t_init,
-        "warm_total": wt, "warm_main": wm, "y": y,
+        "warm_total": wt, "warm_main": wm, "floor": fl, "y": y,
     }
 
 
@@ -114,7 +116,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64")
     ap.add_argument("--items-per-lane", type=int, default=0)
-    ap.add_argument("--waves-per-block", type=int, default=4)
+    ap.add_argument("--waves-per-block", type=int, default=1)
     ap.add_argument("--tiles-per-wave", type=int, default=0)
     ap.add_argument("--flush-mb", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
@@ -127,9 +129,10 @@ def main():
         sys.exit(2)
 
     dtype = np.float64 if a.dtype == "f64" else np.float32
-    S = a.items_per_lane or (8 if dtype == np.float64 else 16)
-    tpw = a.tiles_per_wave or (3 if dtype == np.float64 else 2)
-    cfg = (S, a.waves_per_block, tpw, 0)
+    S = a.items_per_lane or (4 if dtype == np.float64 else 8)
+    tpw = a.tiles_per_wave or (6 if dtype == np.float64 else 4)
+    wpb = a.waves_per_block
+    cfg = (S, wpb, tpw, 0)
     if a.workload == "power-law":
         A, x = matgen.power_law_1m(), matgen.vector(102, 1 << 20)
     elif a.workload == "uniform":
@@ -150,6 +153,9 @@ def main():
     achieved = b / (mean_main * 1e-3) / 1e9
     prof = f"segsum_{a.workload.replace('-', '')}_{a.dtype}_traffic.json"
     traffic = load_traffic(prof)
+    floor_ms = float(np.median(r["floor"]))
+    copy_ms = stream_copy(512 << 20, reps=10)
+    copy_gbps = 2 * (512 << 20) / (float(np.median(copy_ms)) * 1e-3) / 1e9
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -166,7 +172,7 @@ def main():
         "config": {
             "workload": A.name, "m": A.m, "n": A.n, "nnz": A.nnz,
             "effective_bytes": b, "kernel": "speculative segmented sum",
-            "items_per_lane": S, "waves_per_block": a.waves_per_block, "tiles_per_wave": tpw,
+            "items_per_lane": S, "waves_per_block": wpb, "tiles_per_wave": tpw,
             "parallelism": "single GPU",
         },
         "pct_hbm_peak": round(100 * value / HBM_PEAK_GBPS, 2),
@@ -176,6 +182,15 @@ def main():
             "traffic": traffic,
             "kernel": f"segsum_kernel<{'double' if vs == 8 else 'float'},{S}>",
             "kernel_mean_us": round(mean_main * 1e3, 3),
+            "measured_ceilings": {
+                "stream_copy_gbps": round(copy_gbps, 1),
+                "matrix_floor_us": round(floor_ms * 1e3, 3),
+                "matrix_floor_gbps": round(b / (floor_ms * 1e-3) / 1e9, 1),
+                "kernel_frac_of_floor": round(floor_ms / mean_main, 4),
+                "note": "floor = same val/col stream and x gathers as the main kernel with no "
+                        "segmentation and no y (bhs_*_bench_floor); on random-column matrices "
+                        "the x gathers, not HBM, bound every CSR SpMV",
+            },
         },
         "timing": {
             "median_step_ms": med_total, "p10_step_ms": float(np.percentile(r["ms_total"], 10)),

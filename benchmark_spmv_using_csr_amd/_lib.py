@@ -82,12 +82,14 @@ _PER_TYPE = {
     "get_stats": [_H, C.POINTER(SpmvStats)],
     "bench": [_H, "V", C.c_int, C.c_int, C.c_longlong, C.c_void_p, C.c_void_p],
     "row_features": [_H, C.c_void_p],
+    "bench_floor": [_H, C.c_int, C.c_longlong, C.c_void_p],
 }
 
 
 def exported_names():
     """Every symbol include/bhsparse_spmv.h declares."""
-    names = ["bhs_error_string", "bhs_last_error", "bhs_device_synchronize", "bhs_device_count"]
+    names = ["bhs_error_string", "bhs_last_error", "bhs_device_synchronize", "bhs_device_count",
+             "bhs_stream_copy"]
     for p in ("s", "d"):
         names += [f"bhs_{p}_{k}" for k in _PER_TYPE]
     return names
@@ -113,6 +115,8 @@ def spmv_lib():
     L.bhs_last_error.restype = C.c_char_p
     L.bhs_device_synchronize.restype = C.c_int
     L.bhs_device_count.restype = C.c_int
+    L.bhs_stream_copy.argtypes = [C.c_int, C.c_longlong, C.c_int, C.c_void_p]
+    L.bhs_stream_copy.restype = C.c_int
     _spmv = L
     return L
 

@@ -67,6 +67,8 @@ struct Impl {
     int sync_cap = 0;
 
     unsigned long long *d_counters = nullptr;   // 8 x u64 scratch for stats/features
+    T *d_floor = nullptr;                       // per-lane partials of the floor probe
+    long long floor_cap = 0;
     void *d_flush = nullptr;
     long long flush_cap = 0;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -159,6 +161,8 @@ void release_data(Impl<T> *im)
     free_ptr(im->d_tile_row);
     free_ptr(im->d_sync_row);
     free_ptr(im->d_sync_val);
+    free_ptr(im->d_floor);
+    im->floor_cap = 0;
     im->tile_cap = im->sync_cap = 0;
     im->an_P = im->an_TPW = 0;
     im->NP = im->NW = im->lead_empty = 0;
@@ -292,15 +296,6 @@ template <typename T, int S, bool STATS>
 void launch_segsum_s(Impl<T> *im, T alpha, unsigned long long *stats)
 {
     bhs::SegsumParams<T> q;
-    q.row_ptr = im->d_row_ptr;
-    q.col = im->d_col;
-    q.val = im->d_val;
-    q.x = im->d_x;
-    q.y = im->d_y;
-    q.tile_row = im->d_tile_row;
-    q.sync_row = im->d_sync_row;
-    q.sync_val = im->d_sync_val;
-    q.stats = stats;
     q.m = im->m;
     q.nnz = im->nnz;
     q.NP = im->NP;
@@ -313,7 +308,8 @@ void launch_segsum_s(Impl<T> *im, T alpha, unsigned long long *stats)
     const int blocks = (im->NW + q.wpb - 1) / q.wpb;
     const size_t lds = (size_t)q.wpb * q.wave_lds_bytes;
     hipLaunchKernelGGL((bhs::segsum_kernel<T, S, STATS>), dim3(blocks), dim3(64 * q.wpb), lds,
-                       im->stream, q);
+                       im->stream, im->d_row_ptr, im->d_col, im->d_val, im->d_x, im->d_y,
+                       im->d_tile_row, im->d_sync_row, im->d_sync_val, stats, q);
 }
 
 template <typename T, bool STATS>
@@ -631,6 +627,68 @@ int bench(bhs_handle_t h, T alpha, int warmup, int reps, long long flush_bytes, 
     return BHSPARSE_SUCCESS;
 }
 
+template <typename T, int S>
+void launch_floor_s(Impl<T> *im)
+{
+    const long long threads = (long long)im->NP * 64;
+    hipLaunchKernelGGL((bhs::floor_kernel<T, S>), dim3((unsigned)((threads + 255) / 256)),
+                       dim3(256), 0, im->stream, im->d_col, im->d_val, im->d_x, im->NP,
+                       im->d_floor);
+}
+
+template <typename T>
+int bench_floor(bhs_handle_t h, int reps, long long flush_bytes, double *ms)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (!im->data) return set_error(BHSPARSE_NOT_INITIALIZED, "init_data first");
+    if (reps < 1 || !ms || flush_bytes < 0)
+        return set_error(BHSPARSE_INVALID_ARGUMENT, "bad reps/output");
+    BHS_HIP(hipSetDevice(im->device));
+    if (int rc = ensure_analysis(im)) return rc;
+    if (im->NP == 0) {
+        for (int i = 0; i < reps; ++i) ms[i] = 0;
+        return BHSPARSE_SUCCESS;
+    }
+    const long long need = (long long)im->NP * 64;
+    if (need > im->floor_cap) {
+        free_ptr(im->d_floor);
+        BHS_HIP(hipMalloc(&im->d_floor, sizeof(T) * (size_t)need));
+        im->floor_cap = need;
+    }
+    const long long n16 = flush_bytes / 16;
+    if (n16 > 0 && flush_bytes > im->flush_cap) {
+        if (im->d_flush) (void)hipFree(im->d_flush);
+        im->d_flush = nullptr;
+        im->flush_cap = 0;
+        BHS_HIP(hipMalloc(&im->d_flush, (size_t)n16 * 16));
+        im->flush_cap = n16 * 16;
+    }
+    auto launch = [&]() {
+        switch (im->cfg.items_per_lane) {
+        case 4: launch_floor_s<T, 4>(im); break;
+        case 8: launch_floor_s<T, 8>(im); break;
+        case 16: launch_floor_s<T, 16>(im); break;
+        default: launch_floor_s<T, 32>(im); break;
+        }
+    };
+    launch();
+    BHS_HIP(hipGetLastError());
+    for (int i = 0; i < reps; ++i) {
+        if (n16 > 0)
+            hipLaunchKernelGGL(bhs::flush_kernel, dim3(4096), dim3(256), 0, im->stream,
+                               reinterpret_cast<bhs::u32x4 *>(im->d_flush), n16, 0u);
+        BHS_HIP(hipEventRecord(im->ev[0], im->stream));
+        launch();
+        BHS_HIP(hipEventRecord(im->ev[1], im->stream));
+        BHS_HIP(hipEventSynchronize(im->ev[1]));
+        float t = 0;
+        BHS_HIP(hipEventElapsedTime(&t, im->ev[0], im->ev[1]));
+        ms[i] = t;
+    }
+    return BHSPARSE_SUCCESS;
+}
+
 template <typename T>
 int row_features(bhs_handle_t h, double *f)
 {
@@ -699,6 +757,51 @@ int bhs_device_count(void)
     return c;
 }
 
+int bhs_stream_copy(int device, long long bytes, int reps, double *ms)
+{
+    if (bytes < 16 || reps < 1 || !ms) return set_error(BHSPARSE_INVALID_ARGUMENT, "bad copy size");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+        return set_error(BHSPARSE_UNSUPPORTED_DEVICE, "no such device");
+    BHS_HIP(hipSetDevice(device));
+    const long long n16 = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = BHSPARSE_SUCCESS;
+    auto fail = [&](hipError_t e, const char *what) {
+        rc = set_error(e == hipErrorOutOfMemory ? BHSPARSE_OUT_OF_MEMORY : BHSPARSE_HIP_ERROR,
+                       std::string(what) + ": " + hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&a, (size_t)n16 * 16)) != hipSuccess) fail(e, "hipMalloc");
+    else if ((e = hipMalloc(&b, (size_t)n16 * 16)) != hipSuccess) fail(e, "hipMalloc");
+    else if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) fail(e, "stream");
+    else if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess)
+        fail(e, "event");
+    else if ((e = hipMemsetAsync(a, 0, (size_t)n16 * 16, st)) != hipSuccess) fail(e, "memset");
+    else {
+        const unsigned blocks = 256 * 16;
+        for (int i = 0; i < reps && rc == BHSPARSE_SUCCESS; ++i) {
+            hipEventRecord(e0, st);
+            hipLaunchKernelGGL(bhs::copy_kernel, dim3(blocks), dim3(256), 0, st,
+                               (const bhs::u32x4 *)a, (bhs::u32x4 *)b, n16);
+            hipEventRecord(e1, st);
+            if ((e = hipEventSynchronize(e1)) != hipSuccess) { fail(e, "copy"); break; }
+            float t = 0;
+            hipEventElapsedTime(&t, e0, e1);
+            ms[i] = t;
+        }
+    }
+    if (st) hipStreamSynchronize(st);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) hipStreamDestroy(st);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    return rc;
+}
+
 #define BHS_DEFINE_API(P, T)                                                                   \
     int bhs_##P##_create(bhs_handle_t *h) { return create<T>(h); }                            \
     int bhs_##P##_init_platform(bhs_handle_t h, int d) { return init_platform<T>(h, d); }    \
@@ -727,7 +830,11 @@ int bhs_device_count(void)
     {                                                                                          \
         return bench<T>(h, a, w, r, fb, mt, mm);                                               \
     }                                                                                          \
-    int bhs_##P##_row_features(bhs_handle_t h, double *f) { return row_features<T>(h, f); }
+    int bhs_##P##_row_features(bhs_handle_t h, double *f) { return row_features<T>(h, f); }    \
+    int bhs_##P##_bench_floor(bhs_handle_t h, int r, long long fb, double *ms)                \
+    {                                                                                          \
+        return bench_floor<T>(h, r, fb, ms);                                                   \
+    }
 
 BHS_DEFINE_API(s, float)
 BHS_DEFINE_API(d, double)

@@ -20,14 +20,15 @@ __device__ __forceinline__ int lane_id()
 
 // Orders this wave's LDS traffic: every LDS write issued before the call is
 // visible to every lane of the wave after it.  (LDS operations of one wave
-// execute in order; the fences stop the compiler from moving memory operations
-// across the point.)  Never a workgroup barrier: waves of a block run
-// independent tile ranges.
+// execute in order; the fences, restricted to the LDS address space, stop the
+// compiler from moving LDS operations across the point while leaving global
+// loads free to be scheduled early.)  Never a workgroup barrier: waves of a
+// block run independent tile ranges.
 __device__ __forceinline__ void wave_sync()
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
     __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
 // Number of set bits of `mask` in lanes below this one.
@@ -55,25 +56,67 @@ __device__ __forceinline__ float read_lane(float v, int l)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
-// Inclusive prefix sum of an int over the 64 lanes.
-__device__ __forceinline__ int wave_incl_scan(int v, int lane)
+// DPP lane moves (GFX9 encodings).  64-bit values move as two 32-bit halves.
+enum : int {
+    kDppXor1 = 0xB1,          // quad_perm [1,0,3,2]
+    kDppXor2 = 0x4E,          // quad_perm [2,3,0,1]
+    kDppRowShr1 = 0x111,
+    kDppRowShr2 = 0x112,
+    kDppRowShr4 = 0x114,
+    kDppRowShr8 = 0x118,
+    kDppRowMirror = 0x140,    // lane i <- lane 15-i inside each row of 16
+    kDppRowHalfMirror = 0x141,// lane i <- lane 7-i inside each half row of 8
+    kDppRowBcast15 = 0x142,   // row r+1 <- lane 15 of row r
+    kDppRowBcast31 = 0x143,   // rows 2, 3 <- lane 31
+};
+
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ int dpp(int v, int old = 0)
 {
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        int t = __shfl_up(v, d, kWave);
-        if (lane >= d) v += t;
-    }
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xF, false);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v)
+{
+    union { double d; int i[2]; } u, r;
+    u.d = v;
+    r.i[0] = __builtin_amdgcn_update_dpp(0, u.i[0], CTRL, 0xF, 0xF, false);
+    r.i[1] = __builtin_amdgcn_update_dpp(0, u.i[1], CTRL, 0xF, 0xF, false);
+    return r.d;
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Inclusive prefix sum of an int over the 64 lanes (DPP row shifts, then the
+// row broadcasts carry the running total across rows of 16).
+__device__ __forceinline__ int wave_incl_scan(int v, int /*lane*/)
+{
+    v += dpp<kDppRowShr1>(v);
+    v += dpp<kDppRowShr2>(v);
+    v += dpp<kDppRowShr4>(v);
+    v += dpp<kDppRowShr8>(v);
+    v += dpp<kDppRowBcast15, 0xA>(v);
+    v += dpp<kDppRowBcast31, 0xC>(v);
     return v;
 }
 
-// Sum over the 64 lanes (xor butterfly; every lane gets the same value
-// because each step adds the same two operands in both partner lanes).
+// Sum over the 64 lanes, the same value in every lane.  Pairwise tree:
+// quads ((v0+v1)+(v2+v3)), then half rows (Q0+Q1), rows (H0+H1) -- each step
+// pairs mirror-image partners, so both partners compute the same sum -- and
+// finally (R0+R1)+(R2+R3) over the four rows.
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
 {
-#pragma unroll
-    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-    return v;
+    v = v + dpp<kDppXor1>(v);
+    v = v + dpp<kDppXor2>(v);
+    v = v + dpp<kDppRowHalfMirror>(v);
+    v = v + dpp<kDppRowMirror>(v);
+    return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
 // Segmented inclusive scan across lanes, left to right.  Lanes whose bit is
@@ -100,9 +143,30 @@ __device__ __forceinline__ T wave_segscan(T v, uint64_t head, int lane)
 // from the caches.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
 __device__ __forceinline__ u32x4 load_nt16(const void *p)
 {
     return __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(p));
+}
+
+__device__ __forceinline__ u32x2 load_nt8(const void *p)
+{
+    return __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(p));
+}
+
+// v if keep, else +0, without a branch (bitwise select keeps loads feeding v
+// unconditional).
+__device__ __forceinline__ double keep_or_zero(double v, bool keep)
+{
+    long long b = __builtin_bit_cast(long long, v);
+    return __builtin_bit_cast(double, keep ? b : 0LL);
+}
+
+__device__ __forceinline__ float keep_or_zero(float v, bool keep)
+{
+    int b = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(float, keep ? b : 0);
 }
 
 // Block index remap so that each XCD (blocks are dealt round-robin over the

@@ -40,50 +40,55 @@ __global__ __launch_bounds__(256) void partition_kernel(const int *__restrict__ 
     tile_row[p] = lo - 1;
 }
 
+// Scalars of one launch (the arrays travel as separate __restrict__ kernel
+// arguments, which lets the compiler use scalar loads for tile_row/row_ptr).
 template <typename T>
 struct SegsumParams {
-    const int *row_ptr;
-    const int *col;
-    const T *val;
-    const T *x;
-    T *y;
-    const int *tile_row;
-    int *sync_row;
-    T *sync_val;
-    unsigned long long *stats;   // [fast, dirty, sync] when counting
     int m, nnz, NP, TPW, NW, lead_empty, wpb, wave_lds_bytes;
     T alpha;
 };
 
-// Loads this lane's S items (e0 .. e0+S-1) and forms val*x[col]; items at or
-// past nnz contribute 0 (kernels_cuda.h:88, :161).
+// One tile's column indices and values in registers.  Loads are coalesced:
+// for load group j, lane l reads G = 16/sizeof(T) consecutive items starting at
+// B + j*64*G + l*G (consecutive lanes read consecutive 16-byte values), so item
+// k = j*G + g of lane l is nonzero B + j*64*G + l*G + g.
 template <typename T, int S>
-__device__ __forceinline__ void load_products(const int *__restrict__ col,
-                                              const T *__restrict__ val,
-                                              const T *__restrict__ x, int e0, int nnz,
-                                              T (&prod)[S])
-{
-    static_assert(S % 4 == 0, "S must be a multiple of 4");
-    constexpr int VPQ = 16 / (int)sizeof(T);   // values per 16-byte load
+struct Tile {
+    static constexpr int G = 16 / (int)sizeof(T);
+    static constexpr int J = S / G;
+    static_assert(S % G == 0 && S >= 4, "S must be a multiple of 16 / sizeof(T) and >= 4");
     int c[S];
     T v[S];
+};
+
+template <typename T, int S>
+__device__ __forceinline__ int tile_item(int B, int lane, int k)
+{
+    constexpr int G = Tile<T, S>::G;
+    return B + (k / G) * (kWave * G) + lane * G + (k % G);
+}
+
+template <typename T, int S>
+__device__ __forceinline__ void tile_load(const int *__restrict__ col, const T *__restrict__ val,
+                                          int B, int lane, Tile<T, S> &t)
+{
+    constexpr int G = Tile<T, S>::G, J = Tile<T, S>::J;
 #pragma unroll
-    for (int i = 0; i < S / 4; ++i) {
-        u32x4 q = load_nt16(col + e0 + 4 * i);
-        c[4 * i + 0] = (int)q.x;
-        c[4 * i + 1] = (int)q.y;
-        c[4 * i + 2] = (int)q.z;
-        c[4 * i + 3] = (int)q.w;
-    }
-#pragma unroll
-    for (int i = 0; i < S / VPQ; ++i) {
-        u32x4 q = load_nt16(val + e0 + VPQ * i);
-        __builtin_memcpy(&v[VPQ * i], &q, 16);
-    }
-#pragma unroll
-    for (int k = 0; k < S; ++k) {
-        const T xv = x[c[k]];
-        prod[k] = (e0 + k < nnz) ? v[k] * xv : T(0);
+    for (int j = 0; j < J; ++j) {
+        const int e = B + j * (kWave * G) + lane * G;
+        const u32x4 qv = load_nt16(val + e);
+        __builtin_memcpy(&t.v[j * G], &qv, 16);
+        if constexpr (G == 4) {
+            const u32x4 qc = load_nt16(col + e);
+            t.c[j * 4 + 0] = (int)qc.x;
+            t.c[j * 4 + 1] = (int)qc.y;
+            t.c[j * 4 + 2] = (int)qc.z;
+            t.c[j * 4 + 3] = (int)qc.w;
+        } else {
+            const u32x2 qc = load_nt8(col + e);
+            t.c[j * 2 + 0] = (int)qc.x;
+            t.c[j * 2 + 1] = (int)qc.y;
+        }
     }
 }
 
@@ -91,6 +96,9 @@ __device__ __forceinline__ void load_products(const int *__restrict__ col,
 // Speculative segmented sum (kernels_cuda.h:7-269, wave64 form).
 //
 // Per tile [B, B+P) with r0 = tile_row[p], r1 = tile_row[p+1]:
+//  * products val*x[col] (kernels_cuda.h:150-162): coalesced loads, the x
+//    gathers all issued before any of them is waited on; the next tile's
+//    loads are in flight while this tile is reduced.
 //  * fast track (r0 == r1, the tile lies inside one row, :79-112): lane sums,
 //    wave sum, added to the carried partial; nothing is stored until the row
 //    closes.
@@ -98,6 +106,8 @@ __device__ __forceinline__ void load_products(const int *__restrict__ col,
 //    - bit flags (:114-148): lanes walk rows [r0, min(r1, m-1)]; a nonempty
 //      row starting at offset o in [0, P) sets bit o%S of lane o/S's word
 //      (LDS atomic OR); an empty row marks the tile dirty and gets y = 0.
+//    - the products go through LDS so lane l owns items [lS, lS+S) in order
+//      (the reference's s_interres transpose, :150-162).
 //    - carry-in (:164-172): if item 0 does not start a row it gets the
 //      carried partial added; lane 0's item 0 always starts a segment.
 //    - lane-sequential segmented reduction over the S items (:174-203);
@@ -118,9 +128,14 @@ __device__ __forceinline__ void load_products(const int *__restrict__ col,
 //    y[idx] += of stale values (bhsparse_spmv_cuda.h:170-173).
 // ---------------------------------------------------------------------------
 template <typename T, int S, bool STATS>
-__global__ __launch_bounds__(512) void segsum_kernel(const SegsumParams<T> q)
+__global__ __launch_bounds__(512) void segsum_kernel(
+    const int *__restrict__ row_ptr, const int *__restrict__ col, const T *__restrict__ val,
+    const T *__restrict__ x, T *__restrict__ y, const int *__restrict__ tile_row,
+    int *__restrict__ sync_row, T *__restrict__ sync_val, unsigned long long *__restrict__ stats,
+    const SegsumParams<T> q)
 {
     constexpr int P = kWave * S;
+    constexpr int G = Tile<T, S>::G;
     extern __shared__ double smem_raw[];
     const int lane = lane_id();
     const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -130,35 +145,68 @@ __global__ __launch_bounds__(512) void segsum_kernel(const SegsumParams<T> q)
 
     unsigned char *wave_lds = reinterpret_cast<unsigned char *>(smem_raw) + wib * q.wave_lds_bytes;
     unsigned *s_head = reinterpret_cast<unsigned *>(wave_lds);
-    T *s_seg = reinterpret_cast<T *>(wave_lds + 256);
+    T *s_buf = reinterpret_cast<T *>(wave_lds + 256);   // products, then segment sums
 
     // Rows before the first nonzero belong to no tile: spread their zeroing.
     if (q.lead_empty > 0) {
         const long long lo = (long long)q.lead_empty * w / q.NW;
         const long long hi = (long long)q.lead_empty * (w + 1) / q.NW;
-        for (long long r = lo + lane; r < hi; r += kWave) q.y[r] = T(0);
+        for (long long r = lo + lane; r < hi; r += kWave) y[r] = T(0);
     }
 
     const int p_begin = w * q.TPW;
     const int p_end = min(p_begin + q.TPW, q.NP);
     const int E0 = p_begin * P;
-    const int r_first = q.tile_row[p_begin];
-    bool syncing = q.row_ptr[r_first] < E0;    // first row began in an earlier wave
+    const int r_first = tile_row[p_begin];
+    bool syncing = row_ptr[r_first] < E0;    // first row began in an earlier wave
     bool did_sync = false;
     bool open = false;                          // a row continues past the last tile
     T carry = T(0);
     unsigned long long n_fast = 0, n_dirty = 0;
 
+    Tile<T, S> cur;
+    tile_load<T, S>(col, val, E0, lane, cur);
+    int r1 = r_first;
+
     for (int p = p_begin; p < p_end; ++p) {
         const int B = p * P;
-        const int r0 = q.tile_row[p];
-        const int r1 = q.tile_row[p + 1];
-        const bool cont = q.row_ptr[r0] < B;    // item 0 continues row r0
+        const int r0 = r1;
+        r1 = tile_row[p + 1];
+        const bool general = r0 != r1;
+
+        // all gathers of this tile in flight at once
+        T xv[S];
+#pragma unroll
+        for (int k = 0; k < S; ++k) xv[k] = x[cur.c[k]];
+
+        // first chunk of the flag walk (rows r0 .. r0+63), loaded up front.
+        // Loads below are unconditional (clamped addresses, always in bounds):
+        // a load under a branch makes the compiler drain every outstanding
+        // load where the branch rejoins.
+        const int stop = (r1 == q.m) ? q.m - 1 : r1;
+        const bool have0 = general && (r0 + lane <= stop);
+        const int ri = min(r0 + lane, q.m - 1);
+        const int rp0 = row_ptr[ri];
+        const int rpn0 = row_ptr[ri + 1];
+        const bool cont = read_lane(rp0, 0) < B;   // item 0 continues row r0 (lane 0 holds row r0)
+
+        // keep the prefetch behind the loads this tile waits for (the vector
+        // memory counter retires in order)
+        __builtin_amdgcn_sched_barrier(0);
+        // next tile's stream in flight while this one is reduced (the last
+        // tile of the wave reloads itself instead of branching)
+        Tile<T, S> nxt;
+        tile_load<T, S>(col, val, (p + 1 < p_end) ? B + P : B, lane, nxt);
 
         T prod[S];
-        load_products<T, S>(q.col, q.val, q.x, B + lane * S, q.nnz, prod);
+        const bool tail = B + P > q.nnz;
+#pragma unroll
+        for (int k = 0; k < S; ++k) {
+            const T pk = cur.v[k] * xv[k];
+            prod[k] = tail ? keep_or_zero(pk, tile_item<T, S>(B, lane, k) < q.nnz) : pk;
+        }
 
-        if (r0 == r1) {                          // fast track
+        if (!general) {                          // fast track
             T s = prod[0];
 #pragma unroll
             for (int k = 1; k < S; ++k) s += prod[k];
@@ -166,31 +214,47 @@ __global__ __launch_bounds__(512) void segsum_kernel(const SegsumParams<T> q)
             carry = cont ? carry + s : s;
             open = true;
             if (STATS) ++n_fast;
+            cur = nxt;
             continue;
         }
 
         // ---- bit flags ----------------------------------------------------
-        const int stop = (r1 == q.m) ? q.m - 1 : r1;
         wave_sync();                             // previous tile's LDS reads are done
         s_head[lane] = 0u;
+#pragma unroll
+        for (int j = 0; j < S / G; ++j) {        // products -> LDS (tile order)
+            T *dst = s_buf + j * (kWave * G) + lane * G;
+#pragma unroll
+            for (int g = 0; g < G; ++g) dst[g] = prod[j * G + g];
+        }
         wave_sync();
-        bool lane_dirty = false;
-        for (int r = r0 + lane; r <= stop; r += kWave) {
-            const int rp = q.row_ptr[r], rpn = q.row_ptr[r + 1];
+        const bool ne0 = have0 && rpn0 > rp0;
+        bool lane_dirty = have0 && !ne0;
+        if (ne0) {
+            const int o = rp0 - B;
+            if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
+        } else if (have0) {
+            y[r0 + lane] = T(0);
+        }
+        for (int r = r0 + kWave + lane; r <= stop; r += kWave) {   // rare: > 64 rows
+            const int rp = row_ptr[r], rpn = row_ptr[r + 1];
             if (rpn > rp) {
                 const int o = rp - B;
                 if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
             } else {
                 lane_dirty = true;
-                q.y[r] = T(0);
+                y[r] = T(0);
             }
         }
         const bool dirty = ballot(lane_dirty) != 0ull;
+        T pc[S];
+#pragma unroll
+        for (int k = 0; k < S; ++k) pc[k] = s_buf[lane * S + k];   // lane-contiguous items
         wave_sync();
         unsigned hb = s_head[lane];
         if (lane == 0) {
             hb |= 1u;
-            if (cont) prod[0] = prod[0] + carry;
+            if (cont) pc[0] = pc[0] + carry;
         }
 
         // ---- segment numbers ---------------------------------------------
@@ -206,91 +270,161 @@ __global__ __launch_bounds__(512) void segsum_kernel(const SegsumParams<T> q)
 #pragma unroll
         for (int k = 0; k < S; ++k) {
             if ((hb >> k) & 1u) {
-                if (seen) { s_seg[sidx] = acc; ++sidx; }
+                if (seen) { s_buf[sidx] = acc; ++sidx; }
                 else { lead = acc; seen = true; }
                 acc = T(0);
             }
-            acc += prod[k];
+            acc += pc[k];
         }
 
         // ---- segments spanning lanes -------------------------------------
         const uint64_t present = ballot(seen);  // bit 0 always set
         const T I = wave_segscan(acc, present, lane);
         const T Iprev = __shfl_up(I, 1, kWave);
-        if (seen && lane > 0) s_seg[seg_base - 1] = Iprev + lead;
-        if (lane == kWave - 1) s_seg[K - 1] = I;
+        if (seen && lane > 0) s_buf[seg_base - 1] = Iprev + lead;
+        if (lane == kWave - 1) s_buf[K - 1] = I;
         wave_sync();
 
         // ---- output ------------------------------------------------------
-        const bool partial_tail = (r1 < q.m) && (q.row_ptr[r1] < B + P);
+        // row_ptr[r1] sits in lane r1-r0 of the first chunk when r1 < r0+64
+        int rp_r1;
+        if (r1 - r0 < kWave) rp_r1 = read_lane(rp0, r1 - r0);
+        else rp_r1 = row_ptr[min(r1, q.m)];
+        const bool partial_tail = (r1 < q.m) && (rp_r1 < B + P);
         const int kb = syncing ? 1 : 0;
         const int ke = partial_tail ? K - 1 : K;
         if (!dirty) {
-            for (int k = kb + lane; k < ke; k += kWave) q.y[r0 + k] = q.alpha * s_seg[k];
+            for (int k = kb + lane; k < ke; k += kWave) y[r0 + k] = q.alpha * s_buf[k];
         } else {
             if (STATS) ++n_dirty;
-            int running = 0;
-            for (int rb = r0; rb <= stop; rb += kWave) {
+            const uint64_t m0 = ballot(ne0);
+            const int rank0 = popc_below(m0);
+            if (ne0 && rank0 >= kb && rank0 < ke) y[r0 + lane] = q.alpha * s_buf[rank0];
+            int running = __popcll(m0);
+            for (int rb = r0 + kWave; rb <= stop; rb += kWave) {
                 const int r = rb + lane;
                 bool ne = false;
-                if (r <= stop) ne = q.row_ptr[r + 1] > q.row_ptr[r];
+                if (r <= stop) ne = row_ptr[r + 1] > row_ptr[r];
                 const uint64_t msk = ballot(ne);
                 const int rank = running + popc_below(msk);
-                if (ne && rank >= kb && rank < ke) q.y[r] = q.alpha * s_seg[rank];
+                if (ne && rank >= kb && rank < ke) y[r] = q.alpha * s_buf[rank];
                 running += __popcll(msk);
             }
         }
         if (syncing) {
             if (lane == 0) {
-                q.sync_row[w] = r0;
-                q.sync_val[w] = q.alpha * s_seg[0];
+                sync_row[w] = r0;
+                sync_val[w] = q.alpha * s_buf[0];
             }
             syncing = false;
             did_sync = true;
         }
-        carry = s_seg[K - 1];
+        carry = s_buf[K - 1];
         open = partial_tail;
+        cur = nxt;
     }
 
     if (open) {
-        const int r_open = q.tile_row[p_end];
+        const int r_open = tile_row[p_end];
         if (syncing) {
             if (lane == 0) {
-                q.sync_row[w] = r_open;
-                q.sync_val[w] = q.alpha * carry;
+                sync_row[w] = r_open;
+                sync_val[w] = q.alpha * carry;
             }
             did_sync = true;
         } else if (lane == 0) {
-            q.y[r_open] = q.alpha * carry;
+            y[r_open] = q.alpha * carry;
         }
     }
-    if (!did_sync && lane == 0) q.sync_row[w] = -1;
+    if (!did_sync && lane == 0) sync_row[w] = -1;
     if (STATS && lane == 0) {
-        atomicAdd(&q.stats[0], n_fast);
-        atomicAdd(&q.stats[1], n_dirty);
-        atomicAdd(&q.stats[2], did_sync ? 1ull : 0ull);
+        atomicAdd(&stats[0], n_fast);
+        atomicAdd(&stats[1], n_dirty);
+        atomicAdd(&stats[2], did_sync ? 1ull : 0ull);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Calibration (bhsparse_spmv_cuda.h:166-173 moved to the device): each row
-// whose partials are spread over several waves is finished by the thread of
-// the first wave in its chain, adding the partials in wave order to the value
-// the row's starting wave stored -- the reference's y[idx] += val loop order.
+// Data-movement floor of a matrix: the main kernel's loads and gathers (same
+// coalesced tile loads, same x gathers) with no segmentation and no y -- one
+// wave per tile, one partial per lane.  Its time bounds what any CSR SpMV of
+// the matrix can reach on the device.
+// ---------------------------------------------------------------------------
+template <typename T, int S>
+__global__ __launch_bounds__(256) void floor_kernel(const int *__restrict__ col,
+                                                   const T *__restrict__ val,
+                                                   const T *__restrict__ x, int NP,
+                                                   T *__restrict__ out)
+{
+    constexpr int P = kWave * S;
+    const int lane = lane_id();
+    const int p = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+    if (p >= NP) return;
+    Tile<T, S> t;
+    tile_load<T, S>(col, val, p * P, lane, t);
+    T xv[S];
+#pragma unroll
+    for (int k = 0; k < S; ++k) xv[k] = x[t.c[k]];
+    T s = t.v[0] * xv[0];
+#pragma unroll
+    for (int k = 1; k < S; ++k) s += t.v[k] * xv[k];
+    out[(long long)p * kWave + lane] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Calibration (bhsparse_spmv_cuda.h:166-173 moved to the device).  Every wave
+// w wrote sync_row[w] (-1: none) and sync_val[w]; consecutive waves with the
+// same row form a chain whose partials must be added, in wave order, to the
+// value the row's starting wave stored.  One wave handles 64 consecutive
+// entries: a segmented scan over runs of equal row gives each run's sum, the
+// lane ending a run adds it to y; a run reaching past the window is extended
+// window by window (wave sums).  Runs continuing from the previous window
+// belong to the wave of that window.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void calibrate_kernel(const int *__restrict__ sync_row,
                                                        const T *__restrict__ sync_val, int NW,
                                                        T *__restrict__ y)
 {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= NW) return;
-    const int r = sync_row[w];
-    if (r < 0) return;
-    if (w > 0 && sync_row[w - 1] == r) return;
-    T acc = y[r];
-    for (int i = w; i < NW && sync_row[i] == r; ++i) acc += sync_val[i];
-    y[r] = acc;
+    const int lane = lane_id();
+    const int base = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * kWave;
+    if (base >= NW) return;
+    const int e = base + lane;
+    const int r = e < NW ? sync_row[e] : -1;
+    const T v = (e < NW && r >= 0) ? sync_val[e] : T(0);
+    const int before = base > 0 ? sync_row[base - 1] : -1;
+    int rprev = __shfl_up(r, 1, kWave);
+    if (lane == 0) rprev = before;
+    const uint64_t starts = ballot(r != rprev) | 1ull;
+    const T s = wave_segscan(v, starts, lane);
+    const int rnext = __shfl_down(r, 1, kWave);
+    const bool run_end = (lane == kWave - 1) || (rnext != r);
+    // the run holding lane 0 continues an earlier window's run?
+    const int r0w = read_lane(r, 0);
+    const bool lead_cont = r0w >= 0 && before == r0w;
+    // run of lane 63 continuing into later windows
+    const int r63 = read_lane(r, kWave - 1);
+    T ext = T(0);
+    if (r63 >= 0) {
+        for (int nb = base + kWave; nb < NW; nb += kWave) {
+            const int e2 = nb + lane;
+            const int r2 = e2 < NW ? sync_row[e2] : -1;
+            const uint64_t same = ballot(r2 == r63);
+            const int len = (same == ~0ull) ? kWave : __builtin_ctzll(~same);
+            if (len == 0) break;
+            const T v2 = lane < len ? sync_val[e2] : T(0);
+            ext = ext + wave_sum(v2);
+            if (len < kWave) break;
+        }
+    }
+    // lane's run starts at the last set bit of `starts` at or below it
+    const uint64_t upto = starts & ((lane == 63) ? ~0ull : ((2ull << lane) - 1));
+    const int run_start = 63 - __builtin_clzll(upto);
+    const bool skip = (run_start == 0) && lead_cont;
+    if (run_end && r >= 0 && !skip) {
+        const T tot = (lane == kWave - 1) ? s + ext : s;
+        y[r] = y[r] + tot;
+    }
 }
 
 template <typename T>
@@ -308,6 +442,15 @@ __global__ __launch_bounds__(256) void flush_kernel(u32x4 *__restrict__ p, long 
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
          i += (long long)gridDim.x * blockDim.x)
         p[i] = z;
+}
+
+// Plain copy of n16 16-byte words (bandwidth probe and counter calibration).
+__global__ __launch_bounds__(256) void copy_kernel(const u32x4 *__restrict__ src,
+                                                  u32x4 *__restrict__ dst, long long n16)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+         i += (long long)gridDim.x * blockDim.x)
+        dst[i] = __builtin_nontemporal_load(src + i);
 }
 
 // ---------------------------------------------------------------------------

@@ -134,6 +134,13 @@ class bhsparse_spmv:  # noqa: N801  (reference class name)
               "bench")
         return t[:reps], k[:reps]
 
+    def bench_floor(self, reps=20, flush_bytes=0) -> np.ndarray:
+        """Per-rep ms of the data-movement floor kernel for this matrix."""
+        ms = np.zeros(max(reps, 1), np.float64)
+        check(self._fn("bench_floor")(self._h, int(reps), int(flush_bytes),
+                                      ms.ctypes.data_as(C.c_void_p)), "bench_floor")
+        return ms[:reps]
+
     def row_features(self) -> np.ndarray:
         f = np.zeros(8, np.float64)
         check(self._fn("row_features")(self._h, f.ctypes.data_as(C.c_void_p)), "row_features")
@@ -154,6 +161,14 @@ class bhsparse_spmv:  # noqa: N801  (reference class name)
 
 def device_count() -> int:
     return spmv_lib().bhs_device_count()
+
+
+def stream_copy(nbytes: int, reps: int = 10, device: int = 0) -> np.ndarray:
+    """Per-rep ms of a device-to-device copy of nbytes (bandwidth probe)."""
+    ms = np.zeros(reps, np.float64)
+    check(spmv_lib().bhs_stream_copy(int(device), int(nbytes), int(reps),
+                                     ms.ctypes.data_as(C.c_void_p)), "stream_copy")
+    return ms
 
 
 def device_synchronize() -> None:

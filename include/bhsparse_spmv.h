@@ -68,6 +68,12 @@ const char *bhs_last_error(void);
 int bhs_device_synchronize(void);
 /* Number of visible devices (0 when none). */
 int bhs_device_count(void);
+/* Device copy bandwidth probe (no reference counterpart): copies `bytes`
+ * between two device buffers `reps` times with a 16-byte-per-lane copy kernel
+ * and writes each copy's HIP-event time in ms to ms[i].  It gives the box's
+ * measured stream-copy ceiling, and under rocprofv3 --pmc the FETCH_SIZE /
+ * WRITE_SIZE reading of a kernel whose traffic is known exactly. */
+int bhs_stream_copy(int device, long long bytes, int reps, double *ms);
 
 /* ---- fp32 build (value_type = float) ---------------------------------- */
 /* new bhsparse_spmv_cuda() -- main.cu:772 */
@@ -119,6 +125,13 @@ int bhs_s_bench(bhs_handle_t h, float alpha, int warmup, int reps, long long flu
 /* GPU reduction over row_ptr/col: 8 row-length features (m, nnz, mean, var,
  * max, empty fraction, bandwidth estimate, precision flag) used by the tuner. */
 int bhs_s_row_features(bhs_handle_t h, double *feat8);
+/* Data-movement floor (no reference counterpart): times a kernel that streams
+ * val and col and gathers x exactly like the main kernel (same tile loads, same
+ * gathers) but does no segmentation and writes no y.  ms[i] per rep; with
+ * flush_bytes > 0 caches are flushed before each rep.  The ratio of this time
+ * to bhs_s_bench's says how close the SpMV is to what the matrix's own memory
+ * traffic allows on the device. */
+int bhs_s_bench_floor(bhs_handle_t h, int reps, long long flush_bytes, double *ms);
 
 /* ---- fp64 build (value_type = double) --------------------------------- */
 int bhs_d_create(bhs_handle_t *h);
@@ -140,6 +153,7 @@ int bhs_d_get_stats(bhs_handle_t h, bhs_spmv_stats *out);
 int bhs_d_bench(bhs_handle_t h, double alpha, int warmup, int reps, long long flush_bytes,
                 double *ms_total, double *ms_main);
 int bhs_d_row_features(bhs_handle_t h, double *feat8);
+int bhs_d_bench_floor(bhs_handle_t h, int reps, long long flush_bytes, double *ms);
 
 #ifdef __cplusplus
 }

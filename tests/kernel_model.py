@@ -22,13 +22,15 @@ def partition(row_ptr, nnz, P):
 
 
 def _wave_sum(v):
+    """wave_sum: quads ((v0+v1)+(v2+v3)), half rows, rows (each step adds the
+    mirror-image partner), then (R0+R1)+(R2+R3); returns the scalar."""
     v = v.copy()
     lanes = np.arange(W)
-    off = W // 2
-    while off > 0:
-        v = v + v[lanes ^ off]
-        off >>= 1
-    return v
+    v = v + v[lanes ^ 1]
+    v = v + v[lanes ^ 2]
+    v = v + v[(lanes & ~7) | (7 - (lanes & 7))]
+    v = v + v[(lanes & ~15) | (15 - (lanes & 15))]
+    return (v[0] + v[16]) + (v[32] + v[48])
 
 
 def _segscan(v, present):
@@ -50,6 +52,44 @@ def _segscan(v, present):
         cov = (cov | (cov << d)) & FULL
         d <<= 1
     return v
+
+
+def _calibrate(sync_row, sync_val, NW, y, T):
+    """calibrate_kernel: one wave per 64 consecutive sync entries."""
+    for base in range(0, NW, W):
+        idx = np.arange(base, base + W)
+        r = np.where(idx < NW, sync_row[np.minimum(idx, NW - 1)], -1)
+        v = np.where((idx < NW) & (r >= 0), sync_val[np.minimum(idx, NW - 1)], T(0)).astype(y.dtype)
+        before = sync_row[base - 1] if base > 0 else -1
+        rprev = np.concatenate([[before], r[:-1]])
+        starts = r != rprev
+        starts[0] = True
+        s = _segscan(v, starts)
+        run_end = np.concatenate([r[1:] != r[:-1], [True]])
+        lead_cont = r[0] >= 0 and before == r[0]
+        ext = T(0)
+        if r[W - 1] >= 0:
+            nb = base + W
+            while nb < NW:
+                i2 = np.arange(nb, nb + W)
+                r2 = np.where(i2 < NW, sync_row[np.minimum(i2, NW - 1)], -1)
+                same = r2 == r[W - 1]
+                ln = W if same.all() else int(np.argmin(same))
+                if ln == 0:
+                    break
+                v2 = np.zeros(W, y.dtype)
+                v2[:ln] = sync_val[nb:nb + ln]
+                ext = T(ext + _wave_sum(v2))
+                if ln < W:
+                    break
+                nb += W
+        run_start = 0
+        for i in range(W):
+            if starts[i]:
+                run_start = i
+            if run_end[i] and r[i] >= 0 and not (run_start == 0 and lead_cont):
+                tot = T(s[i] + ext) if i == W - 1 else s[i]
+                y[r[i]] = T(y[r[i]] + tot)
 
 
 def segsum(row_ptr, col, val, x, S=8, TPW=3, alpha=1.0, dtype=np.float64, counters=None):
@@ -87,6 +127,9 @@ def segsum(row_ptr, col, val, x, S=8, TPW=3, alpha=1.0, dtype=np.float64, counte
     sync_row = np.full(NW, -2, np.int64)
     sync_val = np.zeros(NW, dtype)
     lanes = np.arange(W)
+    G = 16 // np.dtype(dtype).itemsize
+    kk = np.arange(S)
+    coal = (kk[None, :] // G) * (W * G) + lanes[:, None] * G + (kk[None, :] % G)
     for w in range(NW):
         if lead_empty > 0:
             lo = lead_empty * w // NW
@@ -107,11 +150,14 @@ def segsum(row_ptr, col, val, x, S=8, TPW=3, alpha=1.0, dtype=np.float64, counte
             cont = row_ptr[r0] < B
             prod = prod_all[B:B + P].reshape(W, S).copy()
             if r0 == r1:
-                s = prod[:, 0].copy()
+                # coalesced item order: lane l's item k is nonzero
+                # B + (k // G) * 64 * G + l * G + k % G
+                pc = prod_all[B + coal].copy()
+                s = pc[:, 0].copy()
                 for k in range(1, S):
-                    s = s + prod[:, k]
+                    s = s + pc[:, k]
                 s = _wave_sum(s)
-                carry = T(carry + s[0]) if cont else T(s[0])
+                carry = T(carry + s) if cont else T(s)
                 is_open = True
                 st["n_fast"] += 1
                 continue
@@ -187,18 +233,7 @@ def segsum(row_ptr, col, val, x, S=8, TPW=3, alpha=1.0, dtype=np.float64, counte
             sync_row[w] = -1
         if did_sync:
             st["n_sync"] += 1
-    for w in range(NW):
-        r = sync_row[w]
-        if r < 0:
-            continue
-        if w > 0 and sync_row[w - 1] == r:
-            continue
-        a = T(y[r])
-        i = w
-        while i < NW and sync_row[i] == r:
-            a = T(a + sync_val[i])
-            i += 1
-        y[r] = a
+    _calibrate(sync_row, sync_val, NW, y, T)
     if counters is not None:
         counters.update(st)
     return y
