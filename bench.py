@@ -93,7 +93,7 @@ def cpu_baseline(A, x, dtype, y_gpu, budget_s=12.0):
     except Exception:
         pass
     so, used = oracle.time_spmv(A.row_ptr, A.col, v, xx, ncores, 5)
-    reps = int(max(5, min(400, budget_s / max(so, 1e-4))))
+    reps = int(max(5, min(5000, budget_s / max(so, 1e-4))))
     so, used = oracle.time_spmv(A.row_ptr, A.col, v, xx, ncores, reps)
     b = eff_bytes(A.m, A.n, A.nnz, np.dtype(dtype).itemsize)
     v64, x64 = v.astype(np.float64), xx.astype(np.float64)
