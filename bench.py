@@ -37,43 +37,105 @@ def eff_bytes(m, n, nnz, vsize, k=1):
     return nnz * (vsize + 4) + 4 * (m + 1) + k * (n + m) * vsize
 
 
-def load_traffic(name):
+def load_traffic(name, cfg):
     """HBM bytes per launch of the main kernel from the committed rocprofv3
-    --pmc summary for this workload (FETCH_SIZE + WRITE_SIZE, scaled by the copy
-    kernel calibration recorded with it); None if absent."""
+    --pmc summary for this workload and tile (FETCH_SIZE + WRITE_SIZE, scaled by
+    the copy-kernel calibration recorded with it); None if absent."""
     p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
     try:
         with open(p) as f:
             d = json.load(f)
+        if tuple(d.get("config", ())) != tuple(cfg):
+            return None
         return float(d["hbm_bytes_per_launch"])
     except Exception:
         return None
 
 
+def median(a):
+    return float(np.median(a))
+
+
 def run_gpu(A, x, dtype, cfg, warmup, steps, flush_bytes):
+    """Times the workload with the learned tuner's tile (or `cfg` when given),
+    then the fixed default tile and the exhaustive autotuner's best."""
+    from benchmark_spmv_using_csr_amd import tuner
+
     op = bhsparse_spmv(dtype)
     _lib.check(op.init_platform(0), "init_platform")
-    _lib.check(op.set_config(0, *cfg), "set_config")
     y = np.zeros(A.m, dtype)
     t0 = time.perf_counter()
     _lib.check(op.prepare_mem(A.m, A.n, A.nnz, A.row_ptr, A.col, A.val.astype(dtype),
                               x.astype(dtype), y), "prepare_mem")
     t_init = time.perf_counter() - t0
+    if cfg is None:
+        net = tuner.make_net(tuner.load_net())
+        t0 = time.perf_counter()
+        chosen = op.apply_tuner(net, tuner.CLASSES)
+        t_tune = time.perf_counter() - t0
+        cfg = tuner.CLASSES[chosen]
+        source = "learned tuner (data/tuner_net.json)"
+    else:
+        _lib.check(op.set_config(*cfg), "set_config")
+        t_tune = 0.0
+        source = "command line"
     device_synchronize()
     w0 = time.perf_counter()
     ms_total, ms_main = op.bench(1.0, warmup=warmup, reps=steps, flush_bytes=flush_bytes)
     device_synchronize()
     wall = time.perf_counter() - w0
     _lib.check(op.get_y(), "get_y")
-    # warm numbers (no flush) and the matrix's data-movement floor
-    wt, wm = op.bench(1.0, warmup=3, reps=max(10, steps // 2), flush_bytes=0)
-    fl = op.bench_floor(reps=max(10, steps // 2), flush_bytes=flush_bytes)
+    reps2 = max(10, steps // 2)
+    wt, wm = op.bench(1.0, warmup=3, reps=reps2, flush_bytes=0)
+    fl = op.bench_floor(reps=reps2, flush_bytes=flush_bytes)
+    # the method without the tuner, and the exhaustive judge
+    _lib.check(op.set_config(*tuner.CLASSES[tuner.DEFAULT_CLASS]), "set_config")
+    dt, _ = op.bench(1.0, warmup=3, reps=reps2, flush_bytes=flush_bytes)
+    us, best = op.autotune(tuner.CLASSES, reps=9, flush_bytes=flush_bytes)
     op.close()
     return {
+        "cfg": cfg, "cfg_source": source, "tune_s": t_tune,
         "ms_total": ms_total, "ms_main": ms_main, "wall_s": wall, "init_s": t_init,
-        "warm_total": wt, "warm_main": wm, "floor": fl, "y": y,
+        "warm_total": wt, "warm_main": wm, "floor": fl, "default_ms": median(dt),
+        "sweep_us": us, "sweep_best": best, "y": y,
+    }
+
+
+def run_batched():
+    """256 uneven matrices x {1, 4, 16} right-hand sides: per-matrix plain
+    launches vs one persistent launch replayed as a hipGraph, schedule from the
+    batch autotuner."""
+    from benchmark_spmv_using_csr_amd import batched
+
+    t0 = time.perf_counter()
+    mats, ks, Xs = batched.batch_set()
+    b = batched.SpmvBatch(mats, ks, np.float64)
+    b.set_x(Xs)
+    setup = time.perf_counter() - t0
+    plain = b.bench(plain=True, warmup=1, reps=5)
+    ms, best = b.autotune(batched.SCHEDULES, reps=5)
+    sched = batched.SCHEDULES[best]
+    bt = b.bench(plain=False, warmup=2, reps=10)
+    b.run(1.0)
+    yb = b.y()
+    b.run_plain(1.0)
+    yp = b.y(plain=True)
+    identical = all(np.array_equal(u, v) for u, v in zip(yb, yp))
+    nnz = sum(A.nnz for A in mats)
+    byts = sum(eff_bytes(A.m, A.n, A.nnz, 8, k) for A, k in zip(mats, ks))
+    b.close()
+    return {
+        "workload": "256 matrices, m log-uniform [1e3, 2e5], 4 families, k in {1,4,16}, fp64",
+        "n_matrices": len(mats), "nnz": int(nnz), "rhs_total": int(sum(ks)),
+        "plain_ms": round(median(plain), 4), "batched_ms": round(median(bt), 4),
+        "speedup": round(median(plain) / median(bt), 3),
+        "batched_effective_gbps": round(byts / (median(bt) * 1e-3) / 1e9, 1),
+        "schedule": {"items_per_lane": sched[0], "tiles_per_wave": sched[1],
+                     "waves_per_block": sched[2], "grid_blocks": sched[3],
+                     "order": "matrix" if sched[4] == 0 else "most work first"},
+        "batched_equals_plain_bitwise": bool(identical), "setup_s": round(setup, 2),
     }
 
 
@@ -115,11 +177,12 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--dtype", choices=["f64", "f32"], default="f64")
-    ap.add_argument("--items-per-lane", type=int, default=0)
-    ap.add_argument("--waves-per-block", type=int, default=1)
-    ap.add_argument("--tiles-per-wave", type=int, default=0)
+    ap.add_argument("--config", default="",
+                    help="variant,items_per_lane,waves_per_block,tiles_per_wave,tpv "
+                         "(default: the learned tuner's choice)")
     ap.add_argument("--flush-mb", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--workload", choices=["power-law", "uniform", "laplace", "block3"],
                     default="power-law")
     a = ap.parse_args()
@@ -128,11 +191,10 @@ def main():
               "(the multi-GPU nnz sharding is a design note, not built)", file=sys.stderr)
         sys.exit(2)
 
+    from benchmark_spmv_using_csr_amd import tuner
+
     dtype = np.float64 if a.dtype == "f64" else np.float32
-    S = a.items_per_lane or (4 if dtype == np.float64 else 8)
-    tpw = a.tiles_per_wave or (6 if dtype == np.float64 else 4)
-    wpb = a.waves_per_block
-    cfg = (S, wpb, tpw, 0)
+    cfg = tuple(int(v) for v in a.config.split(",")) if a.config else None
     if a.workload == "power-law":
         A, x = matgen.power_law_1m(), matgen.vector(102, 1 << 20)
     elif a.workload == "uniform":
@@ -145,17 +207,22 @@ def main():
         x = matgen.vector(104, A.n)
 
     r = run_gpu(A, x, dtype, cfg, a.warmup, a.steps, a.flush_mb << 20)
+    cfg = r["cfg"]
     vs = np.dtype(dtype).itemsize
     b = eff_bytes(A.m, A.n, A.nnz, vs)
-    med_total = float(np.median(r["ms_total"]))
+    med_total = median(r["ms_total"])
     mean_main = float(np.mean(r["ms_main"]))
     value = b / (med_total * 1e-3) / 1e9
     achieved = b / (mean_main * 1e-3) / 1e9
+    S = cfg[1]
     prof = f"segsum_{a.workload.replace('-', '')}_{a.dtype}_traffic.json"
-    traffic = load_traffic(prof)
-    floor_ms = float(np.median(r["floor"]))
+    traffic = load_traffic(prof, cfg)
+    floor_ms = median(r["floor"])
     copy_ms = stream_copy(512 << 20, reps=10)
-    copy_gbps = 2 * (512 << 20) / (float(np.median(copy_ms)) * 1e-3) / 1e9
+    copy_gbps = 2 * (512 << 20) / (median(copy_ms) * 1e-3) / 1e9
+    us, best = r["sweep_us"], r["sweep_best"]
+    kname = (f"segsum_kernel<{'double' if vs == 8 else 'float'},{S}>" if cfg[0] == 0 else
+             f"csrvec_kernel<{'double' if vs == 8 else 'float'},{cfg[4]}>")
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -171,8 +238,8 @@ def main():
         "data": "synthetic (seeded generator), inputs resident in HBM, L2+MALL flushed per step",
         "config": {
             "workload": A.name, "m": A.m, "n": A.n, "nnz": A.nnz,
-            "effective_bytes": b, "kernel": "speculative segmented sum",
-            "items_per_lane": S, "waves_per_block": wpb, "tiles_per_wave": tpw,
+            "effective_bytes": b, "kernel": "speculative segmented sum" if cfg[0] == 0 else
+            "csr-vector", "tile": tuner.class_name(cfg), "tile_source": r["cfg_source"],
             "parallelism": "single GPU",
         },
         "pct_hbm_peak": round(100 * value / HBM_PEAK_GBPS, 2),
@@ -180,7 +247,7 @@ def main():
             "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
             "traffic": traffic,
-            "kernel": f"segsum_kernel<{'double' if vs == 8 else 'float'},{S}>",
+            "kernel": kname,
             "kernel_mean_us": round(mean_main * 1e3, 3),
             "measured_ceilings": {
                 "stream_copy_gbps": round(copy_gbps, 1),
@@ -192,16 +259,27 @@ def main():
                         "the x gathers, not HBM, bound every CSR SpMV",
             },
         },
+        "tuner": {
+            "chosen": tuner.class_name(cfg), "inference_s": round(r["tune_s"], 5),
+            "default_tile": tuner.class_name(tuner.CLASSES[tuner.DEFAULT_CLASS]),
+            "default_ms": round(r["default_ms"], 5),
+            "tuned_speedup_vs_default": round(r["default_ms"] / med_total, 3),
+            "exhaustive_best": tuner.class_name(tuner.CLASSES[best]),
+            "exhaustive_best_us": round(float(us[best]), 3),
+            "regret_vs_exhaustive": round(med_total * 1e3 / float(us[best]) - 1, 4),
+        },
         "timing": {
             "median_step_ms": med_total, "p10_step_ms": float(np.percentile(r["ms_total"], 10)),
             "p90_step_ms": float(np.percentile(r["ms_total"], 90)),
             "mean_main_kernel_ms": mean_main,
             "wall_s_timed_loop_incl_flush": round(r["wall_s"], 4),
-            "warm_median_step_ms": float(np.median(r["warm_total"])),
-            "warm_gbps": round(b / (float(np.median(r["warm_total"])) * 1e-3) / 1e9, 2),
+            "warm_median_step_ms": median(r["warm_total"]),
+            "warm_gbps": round(b / (median(r["warm_total"]) * 1e-3) / 1e9, 2),
             "init_data_s": round(r["init_s"], 4),
         },
     }
+    if not a.no_batched:
+        out["batched"] = run_batched()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A, x, dtype, r["y"])
     print(json.dumps(out), flush=True)

@@ -48,6 +48,15 @@ class SpmvStats(C.Structure):
                 ("n_tiles", "n_waves", "n_fast", "n_dirty", "n_sync", "n_lead_empty")]
 
 
+class BatchSchedule(C.Structure):
+    _fields_ = [("items_per_lane", C.c_int), ("tiles_per_wave", C.c_int),
+                ("waves_per_block", C.c_int), ("grid_blocks", C.c_int), ("order", C.c_int)]
+
+    def as_tuple(self):
+        return (self.items_per_lane, self.tiles_per_wave, self.waves_per_block,
+                self.grid_blocks, self.order)
+
+
 class Csr(C.Structure):
     _fields_ = [("m", C.c_int), ("n", C.c_int), ("nnz", C.c_int),
                 ("row_ptr", C.POINTER(C.c_int)), ("col", C.POINTER(C.c_int)),
@@ -83,15 +92,51 @@ _PER_TYPE = {
     "bench": [_H, "V", C.c_int, C.c_int, C.c_longlong, C.c_void_p, C.c_void_p],
     "row_features": [_H, C.c_void_p],
     "bench_floor": [_H, C.c_int, C.c_longlong, C.c_void_p],
+    "autotune": [_H, C.POINTER(Config), C.c_int, C.c_int, C.c_longlong, C.c_void_p,
+                 C.POINTER(C.c_int)],
+    "apply_tuner": [_H, C.c_void_p, C.POINTER(Config), C.c_int, C.POINTER(C.c_int)],
+}
+
+_BATCH = {
+    "batch_create": ([C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                      C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], "int"),
+    "batch_set_x": ([C.c_void_p, C.c_void_p], "int"),
+    "batch_set_schedule": ([C.c_void_p, C.POINTER(BatchSchedule)], "int"),
+    "batch_get_schedule": ([C.c_void_p, C.POINTER(BatchSchedule)], "int"),
+    "batch_run": ([C.c_void_p, "V"], "int"),
+    "batch_run_plain": ([C.c_void_p, "V"], "int"),
+    "batch_get_y": ([C.c_void_p, C.c_void_p, C.c_int], "int"),
+    "batch_bench": ([C.c_void_p, "V", C.c_int, C.c_int, C.c_int, C.c_void_p], "int"),
+    "batch_autotune": ([C.c_void_p, C.POINTER(BatchSchedule), C.c_int, C.c_int, C.c_void_p,
+                        C.POINTER(C.c_int)], "int"),
+    "batch_destroy": ([C.c_void_p], "int"),
+}
+
+_MLP = {
+    "bhs_tuner_transform": ([C.c_void_p, C.c_void_p], C.c_int),
+    "bhs_mlp_param_count": ([C.c_int, C.c_int, C.c_int], C.c_longlong),
+    "bhs_mlp_create": ([C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+    "bhs_mlp_destroy": ([C.c_void_p], C.c_int),
+    "bhs_mlp_set_params": ([C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p], C.c_int),
+    "bhs_mlp_get_params": ([C.c_void_p, C.c_void_p, C.c_longlong], C.c_int),
+    "bhs_mlp_forward": ([C.c_void_p, C.c_void_p, C.c_int, C.c_void_p], C.c_int),
+    "bhs_mlp_loss_grad": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                           C.c_void_p], C.c_int),
+    "bhs_mlp_train": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float,
+                       C.c_float, C.c_void_p], C.c_int),
+    "bhs_mlp_loss_grad_f64": ([C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                               C.c_void_p], C.c_int),
 }
 
 
 def exported_names():
     """Every symbol include/bhsparse_spmv.h declares."""
     names = ["bhs_error_string", "bhs_last_error", "bhs_device_synchronize", "bhs_device_count",
-             "bhs_stream_copy"]
+             "bhs_stream_copy"] + list(_MLP)
     for p in ("s", "d"):
         names += [f"bhs_{p}_{k}" for k in _PER_TYPE]
+        names += [f"bhs_{p}_{k}" for k in _BATCH]
     return names
 
 
@@ -117,6 +162,15 @@ def spmv_lib():
     L.bhs_device_count.restype = C.c_int
     L.bhs_stream_copy.argtypes = [C.c_int, C.c_longlong, C.c_int, C.c_void_p]
     L.bhs_stream_copy.restype = C.c_int
+    for p, vt in (("s", C.c_float), ("d", C.c_double)):
+        for k, (args, _) in _BATCH.items():
+            f = getattr(L, f"bhs_{p}_{k}")
+            f.argtypes = [vt if a == "V" else a for a in args]
+            f.restype = C.c_int
+    for name, (args, res) in _MLP.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
     _spmv = L
     return L
 

@@ -19,7 +19,7 @@ ROOT = os.path.dirname(PKG)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["bhs_api.hip"]
+HIP_SOURCES = ["bhs_api.hip", "bhs_mlp.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off",            # products are rounded before they are summed
@@ -62,12 +62,24 @@ def build_host() -> str:
     return out
 
 
+def build_example() -> str:
+    """The C++ drop-in example (reference's `./spmv 0`) against the built library."""
+    out = os.path.join(PKG, "examples", "spmv_small")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}",
+          os.path.join(PKG, "examples", "spmv_small.cpp"), f"-L{PKG}", "-lbhsparse_spmv",
+          f"-Wl,-rpath,{PKG}", "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all() -> None:
     with ThreadPoolExecutor(max_workers=2) as ex:
         fh = ex.submit(build_hip)
         fo = ex.submit(build_host)
         fh.result()
         fo.result()
+    build_example()
 
 
 if __name__ == "__main__":

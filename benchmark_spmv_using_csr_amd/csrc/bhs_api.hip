@@ -7,6 +7,7 @@
 // them, every buffer is plain device memory (hipMalloc), nothing is computed
 // on the host per call, and a call never depends on y's previous contents.
 #include "bhs_kernels.h"
+#include "bhs_tuner.h"
 #include "../../include/bhsparse_spmv.h"
 
 #include <algorithm>
@@ -35,6 +36,14 @@ int set_error(int code, const std::string &msg)
                              std::string(#call) + ": " + hipGetErrorString(e_));         \
     } while (0)
 
+} // namespace
+
+namespace bhs {
+int mlp_set_error(int code, const std::string &msg) { return set_error(code, msg); }
+} // namespace bhs
+
+namespace {
+
 // The largest tile (64 lanes x 32 items) -- arrays are padded to it.
 constexpr int kMaxTile = 64 * 32;
 // nnz limit keeps every tile index, tile boundary and padded index in int.
@@ -55,7 +64,8 @@ struct Impl {
     T *d_val = nullptr, *d_x = nullptr, *d_y = nullptr;
     T *h_y = nullptr;   // borrowed, as _h_y in bhsparse_spmv_cuda.h:248
 
-    bhs_config cfg{BHS_VARIANT_SEGSUM, sizeof(T) == 8 ? 8 : 16, 4, sizeof(T) == 8 ? 3 : 2, 0};
+    // the fixed default tile (the tuner's "no tuner" baseline class)
+    bhs_config cfg{BHS_VARIANT_SEGSUM, 8, 4, 4, 0};
 
     // analysis for the current tile shape
     int an_P = 0, an_TPW = 0;
@@ -69,6 +79,7 @@ struct Impl {
     unsigned long long *d_counters = nullptr;   // 8 x u64 scratch for stats/features
     T *d_floor = nullptr;                       // per-lane partials of the floor probe
     long long floor_cap = 0;
+    int *d_tuner_io = nullptr;                  // tuner inference inputs/outputs (ints)
     void *d_flush = nullptr;
     long long flush_cap = 0;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -491,6 +502,7 @@ int free_platform(bhs_handle_t h)
     im->d_flush = nullptr;
     im->flush_cap = 0;
     free_ptr(im->d_counters);
+    free_ptr(im->d_tuner_io);
     for (auto &e : im->ev)
         if (e) { (void)hipEventDestroy(e); e = nullptr; }
     if (im->stream) (void)hipStreamDestroy(im->stream);
@@ -689,6 +701,91 @@ int bench_floor(bhs_handle_t h, int reps, long long flush_bytes, double *ms)
     return BHSPARSE_SUCCESS;
 }
 
+// Median time (us) of every candidate configuration; the fastest is installed.
+// Candidates that cannot run (LDS limit) get -1.
+template <typename T>
+int autotune(bhs_handle_t h, const bhs_config *cands, int n, int reps, long long flush_bytes,
+             double *us, int *best)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (!im->data) return set_error(BHSPARSE_NOT_INITIALIZED, "init_data first");
+    if (!cands || n < 1 || reps < 1 || !us) return set_error(BHSPARSE_INVALID_ARGUMENT, "bad candidates");
+    for (int i = 0; i < n; ++i)
+        if (int rc = check_config(cands[i])) return rc;
+    const bhs_config keep = im->cfg;
+    std::vector<double> t(reps);
+    int bi = -1;
+    for (int i = 0; i < n; ++i) {
+        us[i] = -1;
+        if (cands[i].variant == BHS_VARIANT_SEGSUM && segsum_lds_bytes<T>(cands[i]) > im->max_lds)
+            continue;
+        im->cfg = cands[i];
+        if (int rc = bench<T>(h, T(1), 2, reps, flush_bytes, t.data(), nullptr)) {
+            im->cfg = keep;
+            return rc;
+        }
+        std::vector<double> s = t;
+        std::nth_element(s.begin(), s.begin() + reps / 2, s.end());
+        us[i] = s[reps / 2] * 1e3;
+        if (bi < 0 || us[i] < us[bi]) bi = i;
+    }
+    if (bi < 0) {
+        im->cfg = keep;
+        return set_error(BHSPARSE_INVALID_ARGUMENT, "no runnable candidate");
+    }
+    im->cfg = cands[bi];
+    if (best) *best = bi;
+    BHS_HIP(hipSetDevice(im->device));
+    return ensure_analysis(im);
+}
+
+// Learned choice: row statistics (GPU reduction) -> features -> MLP -> argmin,
+// all on the device; installs classes[*chosen].
+template <typename T>
+int apply_tuner(bhs_handle_t h, bhs_mlp_t net, const bhs_config *classes, int K, int *chosen)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (!im->data) return set_error(BHSPARSE_NOT_INITIALIZED, "init_data first");
+    if (!net || !classes) return set_error(BHSPARSE_INVALID_ARGUMENT, "null network/classes");
+    const bhs::Mlp *mlp = reinterpret_cast<const bhs::Mlp *>(net);
+    if (mlp->n_out != K || K > bhs::kMaxClasses)
+        return set_error(BHSPARSE_INVALID_ARGUMENT, "class count does not match the network");
+    if (mlp->device != im->device)
+        return set_error(BHSPARSE_INVALID_ARGUMENT, "network lives on another device");
+    BHS_HIP(hipSetDevice(im->device));
+    std::vector<int> io(4 + K);
+    io[0] = im->m;
+    io[1] = im->nnz;
+    io[2] = sizeof(T) == 8;
+    io[3] = -1;
+    for (int k = 0; k < K; ++k) {
+        const bool ok = check_config(classes[k]) == BHSPARSE_SUCCESS &&
+                        !(classes[k].variant == BHS_VARIANT_SEGSUM &&
+                          segsum_lds_bytes<T>(classes[k]) > im->max_lds);
+        io[4 + k] = ok;
+    }
+    if (!im->d_tuner_io) BHS_HIP(hipMalloc(&im->d_tuner_io, sizeof(int) * (4 + bhs::kMaxClasses)));
+    BHS_HIP(hipMemcpyAsync(im->d_tuner_io, io.data(), sizeof(int) * io.size(),
+                           hipMemcpyHostToDevice, im->stream));
+    launch_fill(im, im->d_counters, 8);
+    if (im->m > 0) {
+        const int blocks = (int)std::min<long long>(((long long)im->m + 255) / 256, 2048);
+        hipLaunchKernelGGL(bhs::row_stats_kernel, dim3(blocks), dim3(256), 0, im->stream, im->m,
+                           im->d_row_ptr, im->d_col, im->d_counters);
+    }
+    BHS_HIP(bhs::launch_tuner_infer(im->d_counters, im->d_tuner_io, im->d_tuner_io + 2, 1, *mlp, K,
+                                    im->d_tuner_io + 4, im->d_tuner_io + 3, nullptr, im->stream));
+    int c = -1;
+    BHS_HIP(hipMemcpyAsync(&c, im->d_tuner_io + 3, sizeof(int), hipMemcpyDeviceToHost, im->stream));
+    BHS_HIP(hipStreamSynchronize(im->stream));
+    if (c < 0 || c >= K) return set_error(BHSPARSE_INVALID_ARGUMENT, "no runnable class");
+    im->cfg = classes[c];
+    if (chosen) *chosen = c;
+    return ensure_analysis(im);
+}
+
 template <typename T>
 int row_features(bhs_handle_t h, double *f)
 {
@@ -722,6 +819,8 @@ int row_features(bhs_handle_t h, double *f)
     f[7] = sizeof(T) == 8 ? 1.0 : 0.0;
     return BHSPARSE_SUCCESS;
 }
+
+#include "bhs_batch.inc"
 
 } // namespace
 
@@ -783,22 +882,22 @@ int bhs_stream_copy(int device, long long bytes, int reps, double *ms)
     else {
         const unsigned blocks = 256 * 16;
         for (int i = 0; i < reps && rc == BHSPARSE_SUCCESS; ++i) {
-            hipEventRecord(e0, st);
+            (void)hipEventRecord(e0, st);
             hipLaunchKernelGGL(bhs::copy_kernel, dim3(blocks), dim3(256), 0, st,
                                (const bhs::u32x4 *)a, (bhs::u32x4 *)b, n16);
-            hipEventRecord(e1, st);
+            (void)hipEventRecord(e1, st);
             if ((e = hipEventSynchronize(e1)) != hipSuccess) { fail(e, "copy"); break; }
             float t = 0;
-            hipEventElapsedTime(&t, e0, e1);
+            (void)hipEventElapsedTime(&t, e0, e1);
             ms[i] = t;
         }
     }
-    if (st) hipStreamSynchronize(st);
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    if (st) hipStreamDestroy(st);
-    if (a) hipFree(a);
-    if (b) hipFree(b);
+    if (st) (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
     return rc;
 }
 
@@ -834,9 +933,56 @@ int bhs_stream_copy(int device, long long bytes, int reps, double *ms)
     int bhs_##P##_bench_floor(bhs_handle_t h, int r, long long fb, double *ms)                \
     {                                                                                          \
         return bench_floor<T>(h, r, fb, ms);                                                   \
+    }                                                                                          \
+    int bhs_##P##_autotune(bhs_handle_t h, const bhs_config *c, int n, int r, long long fb,    \
+                           double *us, int *best)                                              \
+    {                                                                                          \
+        return autotune<T>(h, c, n, r, fb, us, best);                                          \
+    }                                                                                          \
+    int bhs_##P##_apply_tuner(bhs_handle_t h, bhs_mlp_t net, const bhs_config *c, int k,      \
+                              int *chosen)                                                     \
+    {                                                                                          \
+        return apply_tuner<T>(h, net, c, k, chosen);                                           \
     }
 
 BHS_DEFINE_API(s, float)
 BHS_DEFINE_API(d, double)
+
+#define BHS_DEFINE_BATCH_API(P, T)                                                             \
+    int bhs_##P##_batch_create(bhs_batch_t *b, int device, int n_mats, const int *m,            \
+                               const int *n, const int *nnz, const int *const *row_ptr,         \
+                               const int *const *col, const T *const *val, const int *k)        \
+    {                                                                                          \
+        return batch_create<T>(b, device, n_mats, m, n, nnz, row_ptr, col, val, k);            \
+    }                                                                                          \
+    int bhs_##P##_batch_set_x(bhs_batch_t b, const T *const *X) { return batch_set_x<T>(b, X); } \
+    int bhs_##P##_batch_set_schedule(bhs_batch_t b, const bhs_batch_schedule *s)               \
+    {                                                                                          \
+        return batch_set_schedule<T>(b, s);                                                    \
+    }                                                                                          \
+    int bhs_##P##_batch_get_schedule(bhs_batch_t b, bhs_batch_schedule *s)                     \
+    {                                                                                          \
+        return batch_get_schedule<T>(b, s);                                                    \
+    }                                                                                          \
+    int bhs_##P##_batch_run(bhs_batch_t b, T alpha) { return batch_run<T>(b, alpha, 0); }     \
+    int bhs_##P##_batch_run_plain(bhs_batch_t b, T alpha) { return batch_run<T>(b, alpha, 1); } \
+    int bhs_##P##_batch_get_y(bhs_batch_t b, T *const *Y, int plain)                           \
+    {                                                                                          \
+        return batch_get_y<T>(b, Y, plain);                                                    \
+    }                                                                                          \
+    int bhs_##P##_batch_bench(bhs_batch_t b, T alpha, int plain, int warmup, int reps,         \
+                              double *ms)                                                      \
+    {                                                                                          \
+        return batch_bench<T>(b, alpha, plain, warmup, reps, ms);                              \
+    }                                                                                          \
+    int bhs_##P##_batch_autotune(bhs_batch_t b, const bhs_batch_schedule *c, int n, int reps, \
+                                 double *ms, int *best)                                        \
+    {                                                                                          \
+        return batch_autotune<T>(b, c, n, reps, ms, best);                                     \
+    }                                                                                          \
+    int bhs_##P##_batch_destroy(bhs_batch_t b) { return batch_destroy<T>(b); }
+
+BHS_DEFINE_BATCH_API(s, float)
+BHS_DEFINE_BATCH_API(d, double)
 
 } // extern "C"

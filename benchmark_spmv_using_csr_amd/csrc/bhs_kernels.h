@@ -127,45 +127,73 @@ __device__ __forceinline__ void tile_load(const int *__restrict__ col, const T *
 //    overwrites y completely (beta = 0, repeatable), unlike the reference's
 //    y[idx] += of stale values (bhsparse_spmv_cuda.h:170-173).
 // ---------------------------------------------------------------------------
-template <typename T, int S, bool STATS>
-__global__ __launch_bounds__(512) void segsum_kernel(
-    const int *__restrict__ row_ptr, const int *__restrict__ col, const T *__restrict__ val,
-    const T *__restrict__ x, T *__restrict__ y, const int *__restrict__ tile_row,
-    int *__restrict__ sync_row, T *__restrict__ sync_val, unsigned long long *__restrict__ stats,
-    const SegsumParams<T> q)
+// One matrix as one wave sees it.  X is n x kstride and Y is m x kstride,
+// row-major (a column's right-hand sides share cache lines); the wave computes
+// k of them (k <= KMAX).  Rows' partials that cross waves go to
+// sync_row[w] = key_base + row * kstride (the y slot of RHS 0 of the row) and
+// sync_val[w * KMAX + rr].
+template <typename T>
+struct MatView {
+    const int *row_ptr;
+    const int *tile_row;
+    const int *col;
+    const T *val;
+    const T *x;
+    T *y;
+    int *sync_row;
+    T *sync_val;
+    int m, nnz, NP, NW, lead_empty, k, kstride, TPW, key_base;
+};
+
+template <typename T, int S, int KMAX, bool STRIDED, bool STATS>
+__device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, const int lane,
+                                            unsigned *s_head, T *s_buf, T *s_carry, const T alpha,
+                                            unsigned long long *stats)
 {
     constexpr int P = kWave * S;
     constexpr int G = Tile<T, S>::G;
-    extern __shared__ double smem_raw[];
-    const int lane = lane_id();
-    const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int blk = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    const int w = blk * q.wpb + wib;
-    if (w >= q.NW) return;
-
-    unsigned char *wave_lds = reinterpret_cast<unsigned char *>(smem_raw) + wib * q.wave_lds_bytes;
-    unsigned *s_head = reinterpret_cast<unsigned *>(wave_lds);
-    T *s_buf = reinterpret_cast<T *>(wave_lds + 256);   // products, then segment sums
+    const int k = (KMAX == 1) ? 1 : mv.k;
+    const int ks = STRIDED ? mv.kstride : 1;
+    const int *__restrict__ row_ptr = mv.row_ptr;
+    const int *__restrict__ tile_row = mv.tile_row;
+    const T *__restrict__ x = mv.x;
+    T *__restrict__ y = mv.y;
 
     // Rows before the first nonzero belong to no tile: spread their zeroing.
-    if (q.lead_empty > 0) {
-        const long long lo = (long long)q.lead_empty * w / q.NW;
-        const long long hi = (long long)q.lead_empty * (w + 1) / q.NW;
-        for (long long r = lo + lane; r < hi; r += kWave) y[r] = T(0);
+    if (mv.lead_empty > 0) {
+        const long long lo = (long long)mv.lead_empty * w / mv.NW;
+        const long long hi = (long long)mv.lead_empty * (w + 1) / mv.NW;
+        for (long long r = lo + lane; r < hi; r += kWave)
+            for (int rr = 0; rr < k; ++rr) y[r * ks + rr] = T(0);
     }
 
-    const int p_begin = w * q.TPW;
-    const int p_end = min(p_begin + q.TPW, q.NP);
+    const int p_begin = w * mv.TPW;
+    const int p_end = min(p_begin + mv.TPW, mv.NP);
     const int E0 = p_begin * P;
     const int r_first = tile_row[p_begin];
     bool syncing = row_ptr[r_first] < E0;    // first row began in an earlier wave
     bool did_sync = false;
     bool open = false;                          // a row continues past the last tile
-    T carry = T(0);
+    T carry1 = T(0);                            // the open row's partial (KMAX == 1)
+    if (KMAX > 1) {
+        if (lane < k) s_carry[lane] = T(0);
+        wave_sync();
+    }
+    auto carry_get = [&](int rr) -> T {
+        if constexpr (KMAX == 1) return carry1;
+        else return s_carry[rr];
+    };
+    auto carry_set = [&](int rr, T v) {
+        if constexpr (KMAX == 1) carry1 = v;
+        else {
+            if (lane == 0) s_carry[rr] = v;
+            wave_sync();
+        }
+    };
     unsigned long long n_fast = 0, n_dirty = 0;
 
     Tile<T, S> cur;
-    tile_load<T, S>(col, val, E0, lane, cur);
+    tile_load<T, S>(mv.col, mv.val, E0, lane, cur);
     int r1 = r_first;
 
     for (int p = p_begin; p < p_end; ++p) {
@@ -174,18 +202,18 @@ __global__ __launch_bounds__(512) void segsum_kernel(
         r1 = tile_row[p + 1];
         const bool general = r0 != r1;
 
-        // all gathers of this tile in flight at once
+        // all gathers of this tile (first right-hand side) in flight at once
         T xv[S];
 #pragma unroll
-        for (int k = 0; k < S; ++k) xv[k] = x[cur.c[k]];
+        for (int kk = 0; kk < S; ++kk) xv[kk] = x[(long long)cur.c[kk] * ks];
 
         // first chunk of the flag walk (rows r0 .. r0+63), loaded up front.
         // Loads below are unconditional (clamped addresses, always in bounds):
         // a load under a branch makes the compiler drain every outstanding
         // load where the branch rejoins.
-        const int stop = (r1 == q.m) ? q.m - 1 : r1;
+        const int stop = (r1 == mv.m) ? mv.m - 1 : r1;
         const bool have0 = general && (r0 + lane <= stop);
-        const int ri = min(r0 + lane, q.m - 1);
+        const int ri = min(r0 + lane, mv.m - 1);
         const int rp0 = row_ptr[ri];
         const int rpn0 = row_ptr[ri + 1];
         const bool cont = read_lane(rp0, 0) < B;   // item 0 continues row r0 (lane 0 holds row r0)
@@ -196,37 +224,36 @@ __global__ __launch_bounds__(512) void segsum_kernel(
         // next tile's stream in flight while this one is reduced (the last
         // tile of the wave reloads itself instead of branching)
         Tile<T, S> nxt;
-        tile_load<T, S>(col, val, (p + 1 < p_end) ? B + P : B, lane, nxt);
-
-        T prod[S];
-        const bool tail = B + P > q.nnz;
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-            const T pk = cur.v[k] * xv[k];
-            prod[k] = tail ? keep_or_zero(pk, tile_item<T, S>(B, lane, k) < q.nnz) : pk;
-        }
+        tile_load<T, S>(mv.col, mv.val, (p + 1 < p_end) ? B + P : B, lane, nxt);
+        const bool tail = B + P > mv.nnz;
 
         if (!general) {                          // fast track
-            T s = prod[0];
+            for (int rr = 0; rr < k; ++rr) {
+                if (rr > 0) {
 #pragma unroll
-            for (int k = 1; k < S; ++k) s += prod[k];
-            s = wave_sum(s);
-            carry = cont ? carry + s : s;
+                    for (int kk = 0; kk < S; ++kk) xv[kk] = x[(long long)cur.c[kk] * ks + rr];
+                }
+                T prod[S];
+#pragma unroll
+                for (int kk = 0; kk < S; ++kk) {
+                    const T pk = cur.v[kk] * xv[kk];
+                    prod[kk] = tail ? keep_or_zero(pk, tile_item<T, S>(B, lane, kk) < mv.nnz) : pk;
+                }
+                T s = prod[0];
+#pragma unroll
+                for (int kk = 1; kk < S; ++kk) s += prod[kk];
+                s = wave_sum(s);
+                carry_set(rr, cont ? carry_get(rr) + s : s);
+            }
             open = true;
             if (STATS) ++n_fast;
             cur = nxt;
             continue;
         }
 
-        // ---- bit flags ----------------------------------------------------
+        // ---- bit flags (once for all right-hand sides) ----------------------
         wave_sync();                             // previous tile's LDS reads are done
         s_head[lane] = 0u;
-#pragma unroll
-        for (int j = 0; j < S / G; ++j) {        // products -> LDS (tile order)
-            T *dst = s_buf + j * (kWave * G) + lane * G;
-#pragma unroll
-            for (int g = 0; g < G; ++g) dst[g] = prod[j * G + g];
-        }
         wave_sync();
         const bool ne0 = have0 && rpn0 > rp0;
         bool lane_dirty = have0 && !ne0;
@@ -234,7 +261,7 @@ __global__ __launch_bounds__(512) void segsum_kernel(
             const int o = rp0 - B;
             if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
         } else if (have0) {
-            y[r0 + lane] = T(0);
+            for (int rr = 0; rr < k; ++rr) y[(long long)(r0 + lane) * ks + rr] = T(0);
         }
         for (int r = r0 + kWave + lane; r <= stop; r += kWave) {   // rare: > 64 rows
             const int rp = row_ptr[r], rpn = row_ptr[r + 1];
@@ -243,83 +270,105 @@ __global__ __launch_bounds__(512) void segsum_kernel(
                 if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
             } else {
                 lane_dirty = true;
-                y[r] = T(0);
+                for (int rr = 0; rr < k; ++rr) y[(long long)r * ks + rr] = T(0);
             }
         }
         const bool dirty = ballot(lane_dirty) != 0ull;
-        T pc[S];
-#pragma unroll
-        for (int k = 0; k < S; ++k) pc[k] = s_buf[lane * S + k];   // lane-contiguous items
         wave_sync();
         unsigned hb = s_head[lane];
-        if (lane == 0) {
-            hb |= 1u;
-            if (cont) pc[0] = pc[0] + carry;
-        }
+        if (lane == 0) hb |= 1u;
 
         // ---- segment numbers ---------------------------------------------
         const int nh = __popc(hb);
         const int incl = wave_incl_scan(nh, lane);
         const int seg_base = incl - nh;
         const int K = read_lane(incl, kWave - 1);
-
-        // ---- lane-sequential segmented reduction -------------------------
-        T acc = T(0), lead = T(0);
-        bool seen = false;
-        int sidx = seg_base;
-#pragma unroll
-        for (int k = 0; k < S; ++k) {
-            if ((hb >> k) & 1u) {
-                if (seen) { s_buf[sidx] = acc; ++sidx; }
-                else { lead = acc; seen = true; }
-                acc = T(0);
-            }
-            acc += pc[k];
-        }
-
-        // ---- segments spanning lanes -------------------------------------
-        const uint64_t present = ballot(seen);  // bit 0 always set
-        const T I = wave_segscan(acc, present, lane);
-        const T Iprev = __shfl_up(I, 1, kWave);
-        if (seen && lane > 0) s_buf[seg_base - 1] = Iprev + lead;
-        if (lane == kWave - 1) s_buf[K - 1] = I;
-        wave_sync();
-
-        // ---- output ------------------------------------------------------
         // row_ptr[r1] sits in lane r1-r0 of the first chunk when r1 < r0+64
         int rp_r1;
         if (r1 - r0 < kWave) rp_r1 = read_lane(rp0, r1 - r0);
-        else rp_r1 = row_ptr[min(r1, q.m)];
-        const bool partial_tail = (r1 < q.m) && (rp_r1 < B + P);
+        else rp_r1 = row_ptr[min(r1, mv.m)];
+        const bool partial_tail = (r1 < mv.m) && (rp_r1 < B + P);
         const int kb = syncing ? 1 : 0;
         const int ke = partial_tail ? K - 1 : K;
-        if (!dirty) {
-            for (int k = kb + lane; k < ke; k += kWave) y[r0 + k] = q.alpha * s_buf[k];
-        } else {
-            if (STATS) ++n_dirty;
-            const uint64_t m0 = ballot(ne0);
-            const int rank0 = popc_below(m0);
-            if (ne0 && rank0 >= kb && rank0 < ke) y[r0 + lane] = q.alpha * s_buf[rank0];
-            int running = __popcll(m0);
-            for (int rb = r0 + kWave; rb <= stop; rb += kWave) {
-                const int r = rb + lane;
-                bool ne = false;
-                if (r <= stop) ne = row_ptr[r + 1] > row_ptr[r];
-                const uint64_t msk = ballot(ne);
-                const int rank = running + popc_below(msk);
-                if (ne && rank >= kb && rank < ke) y[r] = q.alpha * s_buf[rank];
-                running += __popcll(msk);
+        const uint64_t m0 = ballot(ne0);
+        const int rank0 = popc_below(m0);
+        if (STATS && dirty) ++n_dirty;
+
+        for (int rr = 0; rr < k; ++rr) {
+            if (rr > 0) {
+#pragma unroll
+                for (int kk = 0; kk < S; ++kk) xv[kk] = x[(long long)cur.c[kk] * ks + rr];
             }
+            T prod[S];
+#pragma unroll
+            for (int kk = 0; kk < S; ++kk) {
+                const T pk = cur.v[kk] * xv[kk];
+                prod[kk] = tail ? keep_or_zero(pk, tile_item<T, S>(B, lane, kk) < mv.nnz) : pk;
+            }
+            // products -> LDS in tile order, back lane-contiguous
+            wave_sync();
+#pragma unroll
+            for (int j = 0; j < S / G; ++j) {
+                T *dst = s_buf + j * (kWave * G) + lane * G;
+#pragma unroll
+                for (int g = 0; g < G; ++g) dst[g] = prod[j * G + g];
+            }
+            wave_sync();
+            T pc[S];
+#pragma unroll
+            for (int kk = 0; kk < S; ++kk) pc[kk] = s_buf[lane * S + kk];
+            wave_sync();
+            if (lane == 0 && cont) pc[0] = pc[0] + carry_get(rr);
+
+            // ---- lane-sequential segmented reduction ---------------------
+            T acc = T(0), lead = T(0);
+            bool seen = false;
+            int sidx = seg_base;
+#pragma unroll
+            for (int kk = 0; kk < S; ++kk) {
+                if ((hb >> kk) & 1u) {
+                    if (seen) { s_buf[sidx] = acc; ++sidx; }
+                    else { lead = acc; seen = true; }
+                    acc = T(0);
+                }
+                acc += pc[kk];
+            }
+
+            // ---- segments spanning lanes ---------------------------------
+            const uint64_t present = ballot(seen);  // bit 0 always set
+            const T I = wave_segscan(acc, present, lane);
+            const T Iprev = __shfl_up(I, 1, kWave);
+            if (seen && lane > 0) s_buf[seg_base - 1] = Iprev + lead;
+            if (lane == kWave - 1) s_buf[K - 1] = I;
+            wave_sync();
+
+            // ---- output ----------------------------------------------------
+            if (!dirty) {
+                for (int kk = kb + lane; kk < ke; kk += kWave)
+                    y[(long long)(r0 + kk) * ks + rr] = alpha * s_buf[kk];
+            } else {
+                if (ne0 && rank0 >= kb && rank0 < ke)
+                    y[(long long)(r0 + lane) * ks + rr] = alpha * s_buf[rank0];
+                int running = __popcll(m0);
+                for (int rb = r0 + kWave; rb <= stop; rb += kWave) {
+                    const int r = rb + lane;
+                    bool ne = false;
+                    if (r <= stop) ne = row_ptr[r + 1] > row_ptr[r];
+                    const uint64_t msk = ballot(ne);
+                    const int rank = running + popc_below(msk);
+                    if (ne && rank >= kb && rank < ke)
+                        y[(long long)r * ks + rr] = alpha * s_buf[rank];
+                    running += __popcll(msk);
+                }
+            }
+            if (syncing && lane == 0) mv.sync_val[(long long)w * KMAX + rr] = alpha * s_buf[0];
+            carry_set(rr, s_buf[K - 1]);
         }
         if (syncing) {
-            if (lane == 0) {
-                sync_row[w] = r0;
-                sync_val[w] = q.alpha * s_buf[0];
-            }
+            if (lane == 0) mv.sync_row[w] = mv.key_base + r0 * ks;
             syncing = false;
             did_sync = true;
         }
-        carry = s_buf[K - 1];
         open = partial_tail;
         cur = nxt;
     }
@@ -328,20 +377,42 @@ __global__ __launch_bounds__(512) void segsum_kernel(
         const int r_open = tile_row[p_end];
         if (syncing) {
             if (lane == 0) {
-                sync_row[w] = r_open;
-                sync_val[w] = q.alpha * carry;
+                mv.sync_row[w] = mv.key_base + r_open * ks;
+                for (int rr = 0; rr < k; ++rr)
+                    mv.sync_val[(long long)w * KMAX + rr] = alpha * carry_get(rr);
             }
             did_sync = true;
         } else if (lane == 0) {
-            y[r_open] = q.alpha * carry;
+            for (int rr = 0; rr < k; ++rr) y[(long long)r_open * ks + rr] = alpha * carry_get(rr);
         }
     }
-    if (!did_sync && lane == 0) sync_row[w] = -1;
+    if (!did_sync && lane == 0) mv.sync_row[w] = -1;
     if (STATS && lane == 0) {
         atomicAdd(&stats[0], n_fast);
         atomicAdd(&stats[1], n_dirty);
         atomicAdd(&stats[2], did_sync ? 1ull : 0ull);
     }
+}
+
+template <typename T, int S, bool STATS>
+__global__ __launch_bounds__(512) void segsum_kernel(
+    const int *__restrict__ row_ptr, const int *__restrict__ col, const T *__restrict__ val,
+    const T *__restrict__ x, T *__restrict__ y, const int *__restrict__ tile_row,
+    int *__restrict__ sync_row, T *__restrict__ sync_val, unsigned long long *__restrict__ stats,
+    const SegsumParams<T> q)
+{
+    extern __shared__ double smem_raw[];
+    const int lane = lane_id();
+    const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int blk = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int w = blk * q.wpb + wib;
+    if (w >= q.NW) return;
+    unsigned char *wave_lds = reinterpret_cast<unsigned char *>(smem_raw) + wib * q.wave_lds_bytes;
+    MatView<T> mv{row_ptr, tile_row, col, val, x, y, sync_row, sync_val,
+                  q.m, q.nnz, q.NP, q.NW, q.lead_empty, 1, 1, q.TPW, 0};
+    segsum_wave<T, S, 1, false, STATS>(mv, w, lane, reinterpret_cast<unsigned *>(wave_lds),
+                                       reinterpret_cast<T *>(wave_lds + 256), nullptr, q.alpha,
+                                       stats);
 }
 
 // ---------------------------------------------------------------------------
@@ -381,18 +452,24 @@ __global__ __launch_bounds__(256) void floor_kernel(const int *__restrict__ col,
 // window by window (wave sums).  Runs continuing from the previous window
 // belong to the wave of that window.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void calibrate_kernel(const int *__restrict__ sync_row,
-                                                       const T *__restrict__ sync_val, int NW,
-                                                       T *__restrict__ y)
+// One window of 64 sync entries [base, base+64) for right-hand side rr.
+// key(e) = sync_row[e] if entry e holds a partial of RHS rr, else -1.
+template <typename T, int KMAX>
+__device__ __forceinline__ void calibrate_window(const int *__restrict__ sync_row,
+                                                 const T *__restrict__ sync_val,
+                                                 const int *__restrict__ sync_k, int NW, int base,
+                                                 int rr, int lane, T *__restrict__ y)
 {
-    const int lane = lane_id();
-    const int base = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * kWave;
-    if (base >= NW) return;
+    auto key = [&](int e) -> int {
+        if (e < 0 || e >= NW) return -1;
+        const int r = sync_row[e];
+        if (KMAX > 1 && r >= 0 && rr >= sync_k[e]) return -1;
+        return r;
+    };
     const int e = base + lane;
-    const int r = e < NW ? sync_row[e] : -1;
-    const T v = (e < NW && r >= 0) ? sync_val[e] : T(0);
-    const int before = base > 0 ? sync_row[base - 1] : -1;
+    const int r = key(e);
+    const T v = r >= 0 ? sync_val[(long long)e * KMAX + rr] : T(0);
+    const int before = key(base - 1);
     int rprev = __shfl_up(r, 1, kWave);
     if (lane == 0) rprev = before;
     const uint64_t starts = ballot(r != rprev) | 1ull;
@@ -408,11 +485,11 @@ __global__ __launch_bounds__(256) void calibrate_kernel(const int *__restrict__ 
     if (r63 >= 0) {
         for (int nb = base + kWave; nb < NW; nb += kWave) {
             const int e2 = nb + lane;
-            const int r2 = e2 < NW ? sync_row[e2] : -1;
+            const int r2 = key(e2);
             const uint64_t same = ballot(r2 == r63);
             const int len = (same == ~0ull) ? kWave : __builtin_ctzll(~same);
             if (len == 0) break;
-            const T v2 = lane < len ? sync_val[e2] : T(0);
+            const T v2 = lane < len ? sync_val[(long long)e2 * KMAX + rr] : T(0);
             ext = ext + wave_sum(v2);
             if (len < kWave) break;
         }
@@ -423,8 +500,100 @@ __global__ __launch_bounds__(256) void calibrate_kernel(const int *__restrict__ 
     const bool skip = (run_start == 0) && lead_cont;
     if (run_end && r >= 0 && !skip) {
         const T tot = (lane == kWave - 1) ? s + ext : s;
-        y[r] = y[r] + tot;
+        y[(long long)r + rr] = y[(long long)r + rr] + tot;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Calibration (bhsparse_spmv_cuda.h:166-173 moved to the device).  Every wave
+// w wrote sync_row[w] (-1: none) and sync_val[w]; consecutive waves with the
+// same row form a chain whose partials must be added, in wave order, to the
+// value the row's starting wave stored.  One wave handles 64 consecutive
+// entries: a segmented scan over runs of equal row gives each run's sum, the
+// lane ending a run adds it to y; a run reaching past the window is extended
+// window by window (wave sums).  Runs continuing from the previous window
+// belong to the wave of that window.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void calibrate_kernel(const int *__restrict__ sync_row,
+                                                       const T *__restrict__ sync_val, int NW,
+                                                       T *__restrict__ y)
+{
+    const int lane = lane_id();
+    const int base = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * kWave;
+    if (base >= NW) return;
+    calibrate_window<T, 1>(sync_row, sync_val, nullptr, NW, base, 0, lane, y);
+}
+
+// ---------------------------------------------------------------------------
+// Batched solves: many independent CSR matrices with up to KMAX right-hand
+// sides each, in one persistent launch.  Every wave takes (matrix, wave-range)
+// items from a global queue (one atomic per item) until the queue is empty --
+// each wave exits once the counter passes n_items, so the grid always drains.
+// The per-tile arithmetic is segsum_wave's, so a matrix/RHS pair gets exactly
+// the bits a plain launch with the same tile shape gives.
+// ---------------------------------------------------------------------------
+struct BatchMeta {
+    long long rp_off, tr_off, el_off, x_off, y_off;
+    int sync_off, m, nnz, NP, NW, lead_empty, k, pad;
+};
+
+template <typename T, int S, int KMAX>
+__global__ __launch_bounds__(512) void batched_kernel(
+    const int *__restrict__ row_ptr, const int *__restrict__ col, const T *__restrict__ val,
+    const T *__restrict__ x, T *__restrict__ y, const int *__restrict__ tile_row,
+    int *__restrict__ sync_row, T *__restrict__ sync_val, int *__restrict__ sync_k,
+    const BatchMeta *__restrict__ meta, const int2 *__restrict__ items, int n_items,
+    int *__restrict__ queue, int TPW, T alpha, int wave_lds_bytes)
+{
+    extern __shared__ double smem_raw[];
+    const int lane = lane_id();
+    const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    unsigned char *wave_lds = reinterpret_cast<unsigned char *>(smem_raw) + wib * wave_lds_bytes;
+    unsigned *s_head = reinterpret_cast<unsigned *>(wave_lds);
+    T *s_buf = reinterpret_cast<T *>(wave_lds + 256);
+    T *s_carry = s_buf + kWave * S;
+    for (;;) {
+        int it = 0;
+        if (lane == 0) it = atomicAdd(queue, 1);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= n_items) break;
+        const int2 iw = items[it];
+        const BatchMeta md = meta[iw.x];
+        MatView<T> mv{row_ptr + md.rp_off, tile_row + md.tr_off, col + md.el_off,
+                      val + md.el_off, x + md.x_off, y + md.y_off,
+                      sync_row + md.sync_off, sync_val + (long long)md.sync_off * KMAX,
+                      md.m, md.nnz, md.NP, md.NW, md.lead_empty, md.k, md.k, TPW,
+                      (int)md.y_off};
+        segsum_wave<T, S, KMAX, true, false>(mv, iw.y, lane, s_head, s_buf, s_carry, alpha,
+                                             nullptr);
+        if (lane == 0) sync_k[md.sync_off + iw.y] = md.k;
+        wave_sync();                              // LDS is reused by the next item
+    }
+}
+
+// Calibration of a batch: sync entries of each matrix start on a multiple of
+// 64, so every window lies inside one matrix and matches the windows of that
+// matrix's own plain calibration; keys are y slots (unique across the batch).
+template <typename T, int KMAX>
+__global__ __launch_bounds__(256) void calibrate_batched_kernel(const int *__restrict__ sync_row,
+                                                               const T *__restrict__ sync_val,
+                                                               const int *__restrict__ sync_k,
+                                                               int NW, T *__restrict__ y)
+{
+    const int lane = lane_id();
+    const int base = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * kWave;
+    if (base >= NW) return;
+    const int e = base + lane;
+    int kw = (e < NW && sync_row[e] >= 0) ? sync_k[e] : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(kw, off, kWave);
+        kw = o > kw ? o : kw;
+    }
+    kw = __builtin_amdgcn_readfirstlane(kw);
+    for (int rr = 0; rr < kw; ++rr)
+        calibrate_window<T, KMAX>(sync_row, sync_val, sync_k, NW, base, rr, lane, y);
 }
 
 template <typename T>

@@ -1,0 +1,37 @@
+// The reference's built-in example (spmv_cuda/main.cu:431-545, `./spmv 0`)
+// through the C++ operator class: 6 x 6 CSR, x = 1, expected y = [6, 39, 21,
+// 0, 12, 42].  Unlike the reference (which never calls get_y, main.cu:499-531)
+// it reads y back, and runs the SpMV twice to show the result is repeatable.
+// Build: g++ -std=c++17 -I include spmv_small.cpp -L <pkg> -lbhsparse_spmv
+#include <cstdio>
+
+#include "bhsparse_spmv.hpp"
+
+int main()
+{
+    int row_ptr[7] = {0, 3, 9, 11, 11, 12, 15};
+    int col_idx[15] = {0, 2, 5, 0, 1, 2, 3, 4, 5, 2, 4, 4, 2, 3, 4};
+    value_type val[15] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15};
+    value_type x[6] = {1, 1, 1, 1, 1, 1};
+    value_type y[6] = {0, 0, 0, 0, 0, 0};
+    const value_type expect[6] = {6, 39, 21, 0, 12, 42};
+
+    bhsparse_spmv op;
+    int err = op.init_platform();
+    if (err) {
+        std::printf("init_platform failed: %d (%s)\n", err, bhs_last_error());
+        return 1;
+    }
+    err = op.prepare_mem(6, 6, 15, row_ptr, col_idx, val, x, y);
+    int bad = 0;
+    for (int call = 0; call < 2 && !err; ++call) {
+        err = op.run_benchmark();
+        if (!err) err = op.get_y();
+        for (int i = 0; i < 6; ++i) bad += y[i] != expect[i];
+    }
+    std::printf("y = [%g, %g, %g, %g, %g, %g] %s\n", (double)y[0], (double)y[1], (double)y[2],
+                (double)y[3], (double)y[4], (double)y[5], (!err && !bad) ? "PASS" : "FAIL");
+    op.free_platform();
+    op.free_mem();
+    return (err || bad) ? 1 : 0;
+}

@@ -141,6 +141,24 @@ class bhsparse_spmv:  # noqa: N801  (reference class name)
                                       ms.ctypes.data_as(C.c_void_p)), "bench_floor")
         return ms[:reps]
 
+    def autotune(self, classes, reps=7, flush_bytes=1 << 30):
+        """Exhaustive autotuner: (us per class, -1 where a class cannot run;
+        index of the fastest, which is installed)."""
+        arr = (Config * len(classes))(*[Config(*c) for c in classes])
+        us = np.zeros(len(classes), np.float64)
+        best = C.c_int(-1)
+        check(self._fn("autotune")(self._h, arr, len(classes), int(reps), int(flush_bytes),
+                                   us.ctypes.data_as(C.c_void_p), C.byref(best)), "autotune")
+        return us, best.value
+
+    def apply_tuner(self, net, classes) -> int:
+        """Learned choice on the GPU; installs and returns the class index."""
+        arr = (Config * len(classes))(*[Config(*c) for c in classes])
+        chosen = C.c_int(-1)
+        check(self._fn("apply_tuner")(self._h, net.handle, arr, len(classes), C.byref(chosen)),
+              "apply_tuner")
+        return chosen.value
+
     def row_features(self) -> np.ndarray:
         f = np.zeros(8, np.float64)
         check(self._fn("row_features")(self._h, f.ctypes.data_as(C.c_void_p)), "row_features")

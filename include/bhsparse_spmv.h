@@ -37,6 +37,8 @@ extern "C" {
 #define BHS_VARIANT_CSRVEC  1   /* csr-vector (kernels_cuda.h:272-342, CUSP v0.4.0) */
 
 typedef struct bhs_handle_opaque *bhs_handle_t;
+/* The tuner's network (a device-resident MLP); see "learned tuner" below. */
+typedef struct bhs_mlp_opaque *bhs_mlp_t;
 
 /* Counters of one segmented-sum launch (tiles of P = 64*S nonzeros, T tiles
  * per wave).  Filled by bhs_*_get_stats. */
@@ -59,6 +61,20 @@ typedef struct {
     int tiles_per_wave;
     int tpv;
 } bhs_config;
+
+/* A batch of independent matrices (bhs_*_batch_*). */
+typedef struct bhs_batch_opaque *bhs_batch_t;
+
+/* Schedule of a batch: tile shape of the segmented sum, waves per block,
+ * persistent grid size (0 = about 16 resident waves per CU) and item order
+ * (0 = matrix order, 1 = most work first). */
+typedef struct {
+    int items_per_lane;
+    int tiles_per_wave;
+    int waves_per_block;
+    int grid_blocks;
+    int order;
+} bhs_batch_schedule;
 
 /* ---- library-wide ------------------------------------------------------ */
 const char *bhs_error_string(int code);
@@ -132,6 +148,17 @@ int bhs_s_row_features(bhs_handle_t h, double *feat8);
  * to bhs_s_bench's says how close the SpMV is to what the matrix's own memory
  * traffic allows on the device. */
 int bhs_s_bench_floor(bhs_handle_t h, int reps, long long flush_bytes, double *ms);
+/* Exhaustive autotuner (no reference counterpart; the reference fixes its tile
+ * shape at compile time, spmv_cuda/common.h:39-48): times every candidate
+ * (median of `reps` calls, caches flushed when flush_bytes > 0), writes us[i]
+ * (-1 when a candidate cannot run), installs the fastest and returns its index. */
+int bhs_s_autotune(bhs_handle_t h, const bhs_config *cands, int n_cands, int reps,
+                   long long flush_bytes, double *us, int *best);
+/* Learned choice: the 8 row statistics are reduced on the GPU, transformed,
+ * run through `net` on the GPU, and the class with the lowest predicted cost
+ * among those that can run is installed (*chosen = its index). */
+int bhs_s_apply_tuner(bhs_handle_t h, bhs_mlp_t net, const bhs_config *classes, int n_classes,
+                      int *chosen);
 
 /* ---- fp64 build (value_type = double) --------------------------------- */
 int bhs_d_create(bhs_handle_t *h);
@@ -154,6 +181,80 @@ int bhs_d_bench(bhs_handle_t h, double alpha, int warmup, int reps, long long fl
                 double *ms_total, double *ms_main);
 int bhs_d_row_features(bhs_handle_t h, double *feat8);
 int bhs_d_bench_floor(bhs_handle_t h, int reps, long long flush_bytes, double *ms);
+int bhs_d_autotune(bhs_handle_t h, const bhs_config *cands, int n_cands, int reps,
+                   long long flush_bytes, double *us, int *best);
+int bhs_d_apply_tuner(bhs_handle_t h, bhs_mlp_t net, const bhs_config *classes, int n_classes,
+                      int *chosen);
+
+/* ---- batched solves (no reference counterpart: the reference runs one
+ * matrix and one right-hand side per process, spmv_cuda/main.cu:813-816) ---- */
+/* Copies n_mats CSR matrices (validated like init_data); matrix i gets k[i]
+ * (1..16) right-hand sides.  Default schedule {8, 4, 4, 0, 1}. */
+int bhs_d_batch_create(bhs_batch_t *b, int device, int n_mats, const int *m, const int *n,
+                       const int *nnz, const int *const *row_ptr, const int *const *col,
+                       const double *const *val, const int *k);
+/* X[i] is n[i] x k[i], row-major. */
+int bhs_d_batch_set_x(bhs_batch_t b, const double *const *X);
+int bhs_d_batch_set_schedule(bhs_batch_t b, const bhs_batch_schedule *s);
+int bhs_d_batch_get_schedule(bhs_batch_t b, bhs_batch_schedule *s);
+/* Y_i = alpha * A_i * X_i for every matrix: one hipGraph replay of
+ * [queue reset] -> [persistent kernel over a global (matrix, tile range) queue]
+ * -> [calibration].  Blocking. */
+int bhs_d_batch_run(bhs_batch_t b, double alpha);
+/* The baseline: one plain SpMV (main kernel + calibration) per matrix per
+ * right-hand side, back to back on one stream, on contiguous vectors. */
+int bhs_d_batch_run_plain(bhs_batch_t b, double alpha);
+/* Y[i] (m[i] x k[i], row-major) from the batched (plain = 0) or baseline run. */
+int bhs_d_batch_get_y(bhs_batch_t b, double *const *Y, int plain);
+/* HIP-event times (ms) of `reps` batched (plain = 0) or baseline runs. */
+int bhs_d_batch_bench(bhs_batch_t b, double alpha, int plain, int warmup, int reps, double *ms);
+/* Times every candidate schedule (median of reps), installs the fastest. */
+int bhs_d_batch_autotune(bhs_batch_t b, const bhs_batch_schedule *cands, int n_cands, int reps,
+                         double *ms, int *best);
+int bhs_d_batch_destroy(bhs_batch_t b);
+/* The same in float. */
+int bhs_s_batch_create(bhs_batch_t *b, int device, int n_mats, const int *m, const int *n,
+                       const int *nnz, const int *const *row_ptr, const int *const *col,
+                       const float *const *val, const int *k);
+int bhs_s_batch_set_x(bhs_batch_t b, const float *const *X);
+int bhs_s_batch_set_schedule(bhs_batch_t b, const bhs_batch_schedule *s);
+int bhs_s_batch_get_schedule(bhs_batch_t b, bhs_batch_schedule *s);
+int bhs_s_batch_run(bhs_batch_t b, float alpha);
+int bhs_s_batch_run_plain(bhs_batch_t b, float alpha);
+int bhs_s_batch_get_y(bhs_batch_t b, float *const *Y, int plain);
+int bhs_s_batch_bench(bhs_batch_t b, float alpha, int plain, int warmup, int reps, double *ms);
+int bhs_s_batch_autotune(bhs_batch_t b, const bhs_batch_schedule *cands, int n_cands, int reps,
+                         double *ms, int *best);
+int bhs_s_batch_destroy(bhs_batch_t b);
+
+/* ---- learned tuner (no reference counterpart) -------------------------- */
+/* An MLP n_in(=8) -> hidden -> hidden -> n_out with ReLU, trained on the GPU
+ * by the library's own forward/backward/Adam kernels.  Outputs are predicted
+ * log2(time / best time) per class; the tuner takes the argmin.
+ * Parameters are one float array: W1[h*8], b1[h], W2[h*h], b2[h], W3[k*h], b3[k];
+ * inputs are standardised as (x - mean) / std inside the network. */
+long long bhs_mlp_param_count(int n_in, int n_hidden, int n_out);
+/* raw = {m, nnz, mean len, var len, max len, empty fraction, mean column span,
+ * 1 if fp64}: the transform the tuner applies before standardising. */
+int bhs_tuner_transform(const double *raw8, float *out8);
+int bhs_mlp_create(bhs_mlp_t *net, int device, int n_in, int n_hidden, int n_out);
+int bhs_mlp_destroy(bhs_mlp_t net);
+int bhs_mlp_set_params(bhs_mlp_t net, const float *params, long long n, const float *norm_mean,
+                       const float *norm_std);
+int bhs_mlp_get_params(bhs_mlp_t net, float *params, long long n);
+/* out[n_rows * n_out] for transformed features x[n_rows * 8]. */
+int bhs_mlp_forward(bhs_mlp_t net, const float *x, int n_rows, float *out);
+/* Masked mean squared error against target and its gradient w.r.t. the parameters. */
+int bhs_mlp_loss_grad(bhs_mlp_t net, const float *x, const float *target, const float *mask,
+                      int n_rows, double *loss, float *grad);
+/* `epochs` full-batch Adam steps on the device; loss_hist[epochs] (may be NULL). */
+int bhs_mlp_train(bhs_mlp_t net, const float *x, const float *target, const float *mask,
+                  int n_rows, int epochs, float lr, float weight_decay, double *loss_hist);
+/* The same forward/backward kernels instantiated in double, for gradient checks;
+ * norm = {mean[8], std[8]}. */
+int bhs_mlp_loss_grad_f64(int device, int n_in, int n_hidden, int n_out, const double *params,
+                          const double *norm, const double *x, const double *target,
+                          const double *mask, int n_rows, double *loss, double *grad);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,113 @@
+// bhsparse_spmv.hpp -- header-only C++ operator class over the C ABI of
+// bhsparse_spmv.h, with the method names of the reference's
+// bhsparse_spmv_cuda (spmv_cuda/bhsparse_spmv_cuda.h:7-55) and the north-star
+// names beside them.  value_type follows USE_DOUBLE exactly as in
+// spmv_cuda/common.h:27-31 (default: double).
+//
+//   bhsparse_spmv op;
+//   op.init_platform();                                   // initPlatform()
+//   op.prepare_mem(m, n, nnz, row_ptr, col, val, x, y);   // initData(...)
+//   op.run_benchmark();                                   // spmv(alpha = 1)
+//   op.get_y();                                           // y now holds A*x
+//   op.free_platform(); op.free_mem();
+//
+// Link with libbhsparse_spmv.so.  Every method returns the library's int code
+// (BHSPARSE_SUCCESS = 0) and never exits the process.
+#ifndef BHSPARSE_SPMV_HPP
+#define BHSPARSE_SPMV_HPP
+
+#include "bhsparse_spmv.h"
+
+#ifndef USE_DOUBLE
+#define USE_DOUBLE 1
+#endif
+
+namespace bhsparse {
+
+template <typename T> struct abi;
+
+template <> struct abi<float> {
+    static int create(bhs_handle_t *h) { return bhs_s_create(h); }
+    static int init_platform(bhs_handle_t h, int d) { return bhs_s_init_platform(h, d); }
+    static int init_data(bhs_handle_t h, int m, int n, int nnz, const int *rp, const int *c,
+                         const float *v, const float *x, float *y)
+    {
+        return bhs_s_init_data(h, m, n, nnz, rp, c, v, x, y);
+    }
+    static int set_x(bhs_handle_t h, const float *x) { return bhs_s_set_x(h, x); }
+    static int spmv(bhs_handle_t h, float a) { return bhs_s_spmv(h, a); }
+    static int get_y(bhs_handle_t h) { return bhs_s_get_y(h); }
+    static int free_mem(bhs_handle_t h) { return bhs_s_free_mem(h); }
+    static int free_platform(bhs_handle_t h) { return bhs_s_free_platform(h); }
+    static int destroy(bhs_handle_t h) { return bhs_s_destroy(h); }
+    static int set_config(bhs_handle_t h, const bhs_config *c) { return bhs_s_set_config(h, c); }
+};
+
+template <> struct abi<double> {
+    static int create(bhs_handle_t *h) { return bhs_d_create(h); }
+    static int init_platform(bhs_handle_t h, int d) { return bhs_d_init_platform(h, d); }
+    static int init_data(bhs_handle_t h, int m, int n, int nnz, const int *rp, const int *c,
+                         const double *v, const double *x, double *y)
+    {
+        return bhs_d_init_data(h, m, n, nnz, rp, c, v, x, y);
+    }
+    static int set_x(bhs_handle_t h, const double *x) { return bhs_d_set_x(h, x); }
+    static int spmv(bhs_handle_t h, double a) { return bhs_d_spmv(h, a); }
+    static int get_y(bhs_handle_t h) { return bhs_d_get_y(h); }
+    static int free_mem(bhs_handle_t h) { return bhs_d_free_mem(h); }
+    static int free_platform(bhs_handle_t h) { return bhs_d_free_platform(h); }
+    static int destroy(bhs_handle_t h) { return bhs_d_destroy(h); }
+    static int set_config(bhs_handle_t h, const bhs_config *c) { return bhs_d_set_config(h, c); }
+};
+
+template <typename T>
+class spmv_operator {
+public:
+    spmv_operator() { status_ = abi<T>::create(&h_); }
+    ~spmv_operator()
+    {
+        if (h_) abi<T>::destroy(h_);
+    }
+    spmv_operator(const spmv_operator &) = delete;
+    spmv_operator &operator=(const spmv_operator &) = delete;
+
+    // north-star names
+    int initPlatform(int device = 0) { return status_ ? status_ : abi<T>::init_platform(h_, device); }
+    int initData(int m, int n, int nnzA, int *csrRowPtrA, int *csrColIdxA, T *csrValA, T *x, T *y)
+    {
+        return abi<T>::init_data(h_, m, n, nnzA, csrRowPtrA, csrColIdxA, csrValA, x, y);
+    }
+    int spmv(T alpha = T(1)) { return abi<T>::spmv(h_, alpha); }
+    int setX(const T *x) { return abi<T>::set_x(h_, x); }
+    int setConfig(const bhs_config &c) { return abi<T>::set_config(h_, &c); }
+
+    // reference names (bhsparse_spmv_cuda.h:11-17)
+    int init_platform() { return initPlatform(0); }
+    int prepare_mem(int m, int n, int nnzA, int *csrRowPtrA, int *csrColIdxA, T *csrValA, T *x,
+                    T *y)
+    {
+        return initData(m, n, nnzA, csrRowPtrA, csrColIdxA, csrValA, x, y);
+    }
+    int run_benchmark() { return spmv(T(1)); }
+    int get_y() { return abi<T>::get_y(h_); }
+    int free_platform() { return abi<T>::free_platform(h_); }
+    int free_mem() { return abi<T>::free_mem(h_); }
+
+    bhs_handle_t handle() const { return h_; }
+
+private:
+    bhs_handle_t h_ = nullptr;
+    int status_ = 0;
+};
+
+} // namespace bhsparse
+
+#if USE_DOUBLE
+typedef double value_type;
+#else
+typedef float value_type;
+#endif
+
+typedef bhsparse::spmv_operator<value_type> bhsparse_spmv;
+
+#endif // BHSPARSE_SPMV_HPP

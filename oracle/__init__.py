@@ -69,6 +69,9 @@ def lib():
         L.oracle_rel_error_f64.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p,
                                            C.c_void_p]
         L.oracle_rel_error_f64.restype = C.c_double
+        L.oracle_mlp_loss_grad.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, _f64p, _f64p, _f64p,
+                                           _f64p, C.c_int, C.c_void_p, C.c_void_p]
+        L.oracle_mlp_loss_grad.restype = C.c_double
         _lib = L
     return _lib
 
@@ -171,3 +174,19 @@ def rel_error(row_ptr, col, val, x, y, y_ref, per_row=False):
     e = lib().oracle_rel_error_f64(m, rp, c, v, x, y, yr,
                                    err.ctypes.data_as(C.c_void_p) if per_row else None)
     return (e, err[:m]) if per_row else e
+
+
+def mlp_loss_grad(n_in, h, n_out, params, norm, X, T, M):
+    """FP64 MLP forward + masked MSE + backprop; returns (loss, outputs, grad)."""
+    params = np.ascontiguousarray(params, np.float64)
+    norm = np.ascontiguousarray(norm, np.float64)
+    X = np.ascontiguousarray(X, np.float64)
+    T = np.ascontiguousarray(T, np.float64)
+    M = np.ascontiguousarray(M, np.float64)
+    N = X.shape[0]
+    out = np.empty((N, n_out), np.float64)
+    grad = np.empty(len(params), np.float64)
+    loss = lib().oracle_mlp_loss_grad(n_in, h, n_out, params, norm, X.ravel(), T.ravel(),
+                                      M.ravel(), N, out.ctypes.data_as(C.c_void_p),
+                                      grad.ctypes.data_as(C.c_void_p))
+    return loss, out, grad

@@ -404,3 +404,71 @@ double oracle_rel_error_f64(int m, const int *row_ptr, const int *col,
     }
     return emax;
 }
+
+/* Textbook backpropagation for the tuner's MLP (FP64). */
+double oracle_mlp_loss_grad(int n_in, int h, int n_out, const double *params,
+                            const double *norm, const double *X, const double *T,
+                            const double *M, int N, double *out, double *grad)
+{
+    const double *W1 = params, *b1 = W1 + (size_t)h * n_in, *W2 = b1 + h,
+                 *b2 = W2 + (size_t)h * h, *W3 = b2 + h, *b3 = W3 + (size_t)n_out * h;
+    const size_t P = (size_t)h * n_in + h + (size_t)h * h + h + (size_t)n_out * h + n_out;
+    double *xs = malloc(sizeof(double) * (size_t)n_in);
+    double *a1 = malloc(sizeof(double) * (size_t)h), *a2 = malloc(sizeof(double) * (size_t)h);
+    double *o = malloc(sizeof(double) * (size_t)n_out), *d3 = malloc(sizeof(double) * (size_t)n_out);
+    double *d2 = malloc(sizeof(double) * (size_t)h), *d1 = malloc(sizeof(double) * (size_t)h);
+    double msum = 0, loss = 0;
+    for (size_t i = 0; i < (size_t)N * n_out; i++) msum += M[i];
+    if (msum <= 0) msum = 1;
+    if (grad) for (size_t i = 0; i < P; i++) grad[i] = 0;
+    double *gW1 = grad, *gb1 = grad ? gW1 + (size_t)h * n_in : NULL, *gW2 = grad ? gb1 + h : NULL,
+           *gb2 = grad ? gW2 + (size_t)h * h : NULL, *gW3 = grad ? gb2 + h : NULL,
+           *gb3 = grad ? gW3 + (size_t)n_out * h : NULL;
+    for (int n = 0; n < N; n++) {
+        for (int i = 0; i < n_in; i++) xs[i] = (X[(size_t)n * n_in + i] - norm[i]) / norm[n_in + i];
+        for (int j = 0; j < h; j++) {
+            double a = b1[j];
+            for (int i = 0; i < n_in; i++) a += xs[i] * W1[(size_t)j * n_in + i];
+            a1[j] = a > 0 ? a : 0;
+        }
+        for (int j = 0; j < h; j++) {
+            double a = b2[j];
+            for (int i = 0; i < h; i++) a += a1[i] * W2[(size_t)j * h + i];
+            a2[j] = a > 0 ? a : 0;
+        }
+        for (int k = 0; k < n_out; k++) {
+            double a = b3[k];
+            for (int i = 0; i < h; i++) a += a2[i] * W3[(size_t)k * h + i];
+            o[k] = a;
+            if (out) out[(size_t)n * n_out + k] = a;
+            const double mk = M[(size_t)n * n_out + k], d = a - T[(size_t)n * n_out + k];
+            loss += mk * d * d;
+            d3[k] = 2 * mk * d / msum;
+        }
+        if (!grad) continue;
+        for (int k = 0; k < n_out; k++) {
+            gb3[k] += d3[k];
+            for (int i = 0; i < h; i++) gW3[(size_t)k * h + i] += d3[k] * a2[i];
+        }
+        for (int i = 0; i < h; i++) {
+            double a = 0;
+            for (int k = 0; k < n_out; k++) a += d3[k] * W3[(size_t)k * h + i];
+            d2[i] = a2[i] > 0 ? a : 0;
+        }
+        for (int j = 0; j < h; j++) {
+            gb2[j] += d2[j];
+            for (int i = 0; i < h; i++) gW2[(size_t)j * h + i] += d2[j] * a1[i];
+        }
+        for (int i = 0; i < h; i++) {
+            double a = 0;
+            for (int j = 0; j < h; j++) a += d2[j] * W2[(size_t)j * h + i];
+            d1[i] = a1[i] > 0 ? a : 0;
+        }
+        for (int j = 0; j < h; j++) {
+            gb1[j] += d1[j];
+            for (int i = 0; i < n_in; i++) gW1[(size_t)j * n_in + i] += d1[j] * xs[i];
+        }
+    }
+    free(xs); free(a1); free(a2); free(o); free(d3); free(d2); free(d1);
+    return loss / msum;
+}

@@ -89,6 +89,16 @@ double oracle_rel_error_f64(int m, const int *row_ptr, const int *col,
                             const double *val, const double *x,
                             const double *y, const double *y_ref, double *err);
 
+/* The tuner's MLP n_in -> h -> h -> n_out (ReLU) in FP64: parameters laid out
+ * W1[h*n_in], b1[h], W2[h*h], b2[h], W3[n_out*h], b3[n_out]; inputs
+ * standardised with norm = {mean[n_in], std[n_in]}.  Returns the masked mean
+ * squared error sum(M (O - T)^2) / sum(M) over X[N*n_in]; writes the outputs O
+ * (N*n_out) and the gradient w.r.t. the parameters when the pointers are non-NULL.
+ * No reference counterpart (the tuner is new); pinned by finite differences. */
+double oracle_mlp_loss_grad(int n_in, int h, int n_out, const double *params,
+                            const double *norm, const double *X, const double *T,
+                            const double *M, int N, double *out, double *grad);
+
 #ifdef __cplusplus
 }
 #endif
