@@ -16,22 +16,25 @@ int main()
     value_type y[6] = {0, 0, 0, 0, 0, 0};
     const value_type expect[6] = {6, 39, 21, 0, 12, 42};
 
-    bhsparse_spmv op;
-    int err = op.init_platform();
+    // the reference driver's own lines (main.cu:490-498), unchanged
+    bhsparse_spmv_cuda *bhsparse = new bhsparse_spmv_cuda();
+    int err = bhsparse->init_platform();
     if (err) {
         std::printf("init_platform failed: %d (%s)\n", err, bhs_last_error());
+        delete bhsparse;
         return 1;
     }
-    err = op.prepare_mem(6, 6, 15, row_ptr, col_idx, val, x, y);
+    err = bhsparse->prepare_mem(6, 6, 15, row_ptr, col_idx, val, x, y);
     int bad = 0;
     for (int call = 0; call < 2 && !err; ++call) {
-        err = op.run_benchmark();
-        if (!err) err = op.get_y();
+        err = bhsparse->run_benchmark();
+        if (!err) err = bhsparse->get_y();
         for (int i = 0; i < 6; ++i) bad += y[i] != expect[i];
     }
     std::printf("y = [%g, %g, %g, %g, %g, %g] %s\n", (double)y[0], (double)y[1], (double)y[2],
                 (double)y[3], (double)y[4], (double)y[5], (!err && !bad) ? "PASS" : "FAIL");
-    op.free_platform();
-    op.free_mem();
+    bhsparse->free_platform();
+    bhsparse->free_mem();
+    delete bhsparse;
     return (err || bad) ? 1 : 0;
 }

@@ -109,5 +109,8 @@ typedef float value_type;
 #endif
 
 typedef bhsparse::spmv_operator<value_type> bhsparse_spmv;
+// The reference driver's class name (spmv_cuda/main.cu:772, :490), so that
+// `new bhsparse_spmv_cuda()` compiles unchanged against this header.
+typedef bhsparse_spmv bhsparse_spmv_cuda;
 
 #endif // BHSPARSE_SPMV_HPP

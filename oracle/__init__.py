@@ -72,8 +72,32 @@ def lib():
         L.oracle_mlp_loss_grad.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, _f64p, _f64p, _f64p,
                                            _f64p, C.c_int, C.c_void_p, C.c_void_p]
         L.oracle_mlp_loss_grad.restype = C.c_double
+        L.oracle_set_mutation.argtypes = [C.c_int]
+        L.oracle_set_mutation.restype = None
         _lib = L
     return _lib
+
+
+# Planted mistakes (oracle.h ORACLE_MUT_*), for tests only.
+MUTATIONS = {
+    "serial_drop_last": 1, "serial_sign": 2, "serial_f32_acc": 3, "serial_transpose": 4,
+    "lower_bound": 5, "no_carry": 6, "no_span": 7, "no_calibration": 8, "no_fixup": 9,
+}
+
+
+class mutated:
+    """Context manager: run the oracle with one planted mistake, then restore it."""
+
+    def __init__(self, name):
+        self.kind = MUTATIONS[name]
+
+    def __enter__(self):
+        lib().oracle_set_mutation(self.kind)
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_set_mutation(0)
+        return False
 
 
 def _csr(row_ptr, col, val, dtype=np.float64):

@@ -18,10 +18,40 @@
 #include <omp.h>
 #endif
 
+/* Planted-mistake switch (tests only): 0 = the oracle as written. */
+static int g_mutation = 0;
+void oracle_set_mutation(int kind) { g_mutation = kind; }
+
 /* spmv_cuda/main.cu:755-761 -- the serial loop, accumulating in FP64. */
 void oracle_spmv_serial_f64(int m, const int *row_ptr, const int *col,
                             const double *val, const double *x, double *y)
 {
+    if (g_mutation >= ORACLE_MUT_SERIAL_DROP_LAST && g_mutation <= ORACLE_MUT_SERIAL_TRANSPOSE) {
+        for (int i = 0; i < m; i++) y[i] = 0;
+        for (int i = 0; i < m; i++) {
+            double sum = 0;
+            float sum32 = 0;
+            for (int j = row_ptr[i]; j < row_ptr[i + 1]; j++) {
+                switch (g_mutation) {
+                case ORACLE_MUT_SERIAL_DROP_LAST:
+                    if (j + 1 < row_ptr[i + 1]) sum += x[col[j]] * val[j];
+                    break;
+                case ORACLE_MUT_SERIAL_SIGN:
+                    sum += (j == row_ptr[i] ? -1.0 : 1.0) * x[col[j]] * val[j];
+                    break;
+                case ORACLE_MUT_SERIAL_F32_ACC:
+                    sum32 += (float)(x[col[j]] * val[j]);
+                    break;
+                case ORACLE_MUT_SERIAL_TRANSPOSE:      /* x[i] read as if n == m */
+                    if (col[j] < m && i < m) y[col[j]] += x[i] * val[j];
+                    break;
+                }
+            }
+            if (g_mutation == ORACLE_MUT_SERIAL_F32_ACC) y[i] = sum32;
+            else if (g_mutation != ORACLE_MUT_SERIAL_TRANSPOSE) y[i] = sum;
+        }
+        return;
+    }
     for (int i = 0; i < m; i++) {
         double sum = 0;
         for (int j = row_ptr[i]; j < row_ptr[i + 1]; j++)
@@ -138,29 +168,40 @@ double oracle_time_spmv_f32(int m, const int *row_ptr, const int *col,
 
 /* spmv_cuda/utils_cuda.h:6-34: bisection returning the first index whose
  * row_ptr value exceeds key (std::upper_bound over size entries). */
-int oracle_upper_bound(const int *row_ptr, int size, int key)
+static int upper_bound_impl(const int *row_ptr, int size, int key, int lower)
 {
     int start = 0, stop = size - 1;
     while (stop >= start) {
         int median = start + (stop - start) / 2;
-        if (key >= row_ptr[median]) start = median + 1;
+        if (lower ? key > row_ptr[median] : key >= row_ptr[median]) start = median + 1;
         else stop = median - 1;
     }
     return start;
 }
 
+int oracle_upper_bound(const int *row_ptr, int size, int key)
+{
+    return upper_bound_impl(row_ptr, size, key, g_mutation == ORACLE_MUT_LOWER_BOUND);
+}
+
+/* Row lookups inside the restatement never take the planted mistake (a wrong
+ * row there could index outside y); the mistake reaches the partition and
+ * upper_bound entry points only. */
 static int row_of(const int *row_ptr, int m, int nnz, long long k)
 {
     int key = k > nnz ? nnz : (int)k;          /* clamp, kernels_cuda.h:47 */
-    return oracle_upper_bound(row_ptr, m + 1, key) - 1;
+    return upper_bound_impl(row_ptr, m + 1, key, 0) - 1;
 }
 
 /* spmv_cuda/kernels_cuda.h:39-49 evaluated for every tile boundary. */
 void oracle_partition(int m, int nnz, const int *row_ptr, int P, int *tile_row)
 {
     long long NP = ((long long)nnz + P - 1) / P;
-    for (long long p = 0; p <= NP; p++)
-        tile_row[p] = row_of(row_ptr, m, nnz, p * (long long)P);
+    for (long long p = 0; p <= NP; p++) {
+        long long k = p * (long long)P;
+        int key = k > nnz ? nnz : (int)k;
+        tile_row[p] = oracle_upper_bound(row_ptr, m + 1, key) - 1;
+    }
 }
 
 void oracle_tile_stats_compute(int m, int nnz, const int *row_ptr, int P, int T,
@@ -288,7 +329,7 @@ int oracle_segsum_ref(int W, int S, int T, int m, int nnz, const int *row_ptr,
                 prod[k] = e < nnz ? val[e] * x[col[e]] : 0;
             }
             /* step 3-1: carry into item 0 unless it starts a row (:164-172) */
-            if (!(bf[0] & 1u)) prod[0] = prod[0] + tail;
+            if (!(bf[0] & 1u) && g_mutation != ORACLE_MUT_NO_CARRY) prod[0] = prod[0] + tail;
             bf[0] |= 1u;
             /* step 4-0: lane-sequential segments (:174-203) */
             for (int l = 0; l < W; l++) {
@@ -319,7 +360,7 @@ int oracle_segsum_ref(int W, int S, int T, int m, int nnz, const int *row_ptr,
             /* step 4-1/4-2: serial spanning walk, then add to the trailing slot (:218-235) */
             for (int l = 0; l < W; l++) {
                 double tmp = 0;
-                if (present[l]) {
+                if (present[l] && g_mutation != ORACLE_MUT_NO_SPAN) {
                     tmp = lead_next[l];
                     int id = l + 1;
                     while (!present[id]) { tmp += lead_next[id]; id++; }
@@ -350,10 +391,11 @@ int oracle_segsum_ref(int W, int S, int T, int m, int nnz, const int *row_ptr,
     }
 
     /* step 8: calibration (bhsparse_spmv_cuda.h:170-173) */
-    for (long long b = 0; b < NB; b++) y[sync_idx[b]] += sync_val[b];
+    if (g_mutation != ORACLE_MUT_NO_CALIBRATION)
+        for (long long b = 0; b < NB; b++) y[sync_idx[b]] += sync_val[b];
 
     /* step 9: fix-up of dirty tiles (bhsparse_spmv_cuda.h:184-222) */
-    if (n_spec) {
+    if (n_spec && g_mutation != ORACLE_MUT_NO_FIXUP) {
         double *t = (double *)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
         memcpy(t, y, sizeof(double) * (size_t)m);
         for (long long i = 0; i < n_spec; i++) {

@@ -99,6 +99,23 @@ double oracle_mlp_loss_grad(int n_in, int h, int n_out, const double *params,
                             const double *norm, const double *X, const double *T,
                             const double *M, int N, double *out, double *grad);
 
+/* Planted mistakes, for the tests that show the oracle's checks catch them.
+ * Every other value (0 included) leaves the oracle as written.  Process-wide;
+ * a test sets it and restores 0. */
+enum {
+    ORACLE_MUT_NONE = 0,
+    ORACLE_MUT_SERIAL_DROP_LAST = 1,   /* serial loop skips each row's last term */
+    ORACLE_MUT_SERIAL_SIGN = 2,        /* first term of each row subtracted */
+    ORACLE_MUT_SERIAL_F32_ACC = 3,     /* accumulates in float */
+    ORACLE_MUT_SERIAL_TRANSPOSE = 4,   /* computes A^T x */
+    ORACLE_MUT_LOWER_BOUND = 5,        /* upper_bound becomes lower_bound */
+    ORACLE_MUT_NO_CARRY = 6,           /* segsum_ref: tile carry not added to item 0 */
+    ORACLE_MUT_NO_SPAN = 7,            /* segsum_ref: lane-spanning partials dropped */
+    ORACLE_MUT_NO_CALIBRATION = 8,     /* segsum_ref: synchronizer partials dropped */
+    ORACLE_MUT_NO_FIXUP = 9            /* segsum_ref: dirty tiles not de-speculated */
+};
+void oracle_set_mutation(int kind);
+
 #ifdef __cplusplus
 }
 #endif
