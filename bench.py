@@ -42,7 +42,7 @@ def load_traffic(name, cfg):
     --pmc summary for this workload and tile (FETCH_SIZE + WRITE_SIZE, scaled by
     the copy-kernel calibration recorded with it); None if absent."""
     p = os.path.join(ROOT, "profiles", name)
-    if not os.path.exists(p):
+    if not name or not os.path.exists(p):
         return None
     try:
         with open(p) as f:
@@ -140,6 +140,29 @@ def run_batched():
     }
 
 
+def run_vendor(A, x, dtype, flush_bytes, y_bhs, reps=20):
+    """rocSPARSE's CSR algorithms on the same matrix, timed the same way (cold,
+    HIP events per call): the vendor column of the reference's comparison
+    (cuSPARSE, main.cu:49-183)."""
+    from benchmark_spmv_using_csr_amd import vendor
+
+    b = eff_bytes(A.m, A.n, A.nnz, np.dtype(dtype).itemsize)
+    out = {"library": f"rocSPARSE {vendor.version()} (rocsparse_v2_spmv, CSR, i32 indices)"}
+    algs = {}
+    scale = max(float(np.max(np.abs(y_bhs))), 1e-300)
+    for alg in vendor.ALGS:
+        ms, y, an = vendor.csrmv(A, x.astype(dtype), dtype, alg=alg, warmup=3, reps=reps,
+                                 flush_bytes=flush_bytes)
+        algs[alg] = {"cold_us": round(median(ms) * 1e3, 3),
+                     "gbps": round(b / (median(ms) * 1e-3) / 1e9, 1),
+                     "analysis_ms": round(an, 2),
+                     "max_abs_diff_vs_bhs_over_max_y": float(np.max(np.abs(y - y_bhs)) / scale)}
+    best = min(algs, key=lambda k: algs[k]["cold_us"])
+    out.update({"algs": algs, "best_alg": best, "best_cold_us": algs[best]["cold_us"],
+                "best_gbps": algs[best]["gbps"]})
+    return out
+
+
 def cpu_baseline(A, x, dtype, y_gpu, budget_s=12.0):
     """The reference's CPU paths timed on this host: the serial loop
     (main.cu:755-761) and the OpenMP row-parallel loop (main.cu:377-393) on all
@@ -184,8 +207,12 @@ def main():
     ap.add_argument("--flush-mb", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-vendor", action="store_true")
     ap.add_argument("--workload", choices=["power-law", "uniform", "laplace", "block3"],
                     default="power-law")
+    ap.add_argument("--mtx", default="",
+                    help="Matrix Market file to use instead of a synthetic workload "
+                         "(sorted, duplicates merged; x uniform in [-1, 1])")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported "
@@ -196,7 +223,11 @@ def main():
 
     dtype = np.float64 if a.dtype == "f64" else np.float32
     cfg = tuple(int(v) for v in a.config.split(",")) if a.config else None
-    if a.workload == "power-law":
+    if a.mtx:
+        A = matgen.read_mtx(a.mtx)
+        A.name = f"Matrix Market file {os.path.basename(a.mtx)} m={A.m} n={A.n}"
+        x = matgen.vector(105, A.n)
+    elif a.workload == "power-law":
         A, x = matgen.power_law_1m(), matgen.vector(102, 1 << 20)
     elif a.workload == "uniform":
         A, x = matgen.uniform_65536(), matgen.vector(101, 65536)
@@ -216,14 +247,16 @@ def main():
     value = b / (med_total * 1e-3) / 1e9
     achieved = b / (mean_main * 1e-3) / 1e9
     S = cfg[1]
-    prof = f"segsum_{a.workload.replace('-', '')}_{a.dtype}_traffic.json"
+    prof = "" if a.mtx else f"segsum_{a.workload.replace('-', '')}_{a.dtype}_traffic.json"
     traffic = load_traffic(prof, cfg)
     floor_ms = median(r["floor"])
     copy_ms = stream_copy(512 << 20, reps=10)
     copy_gbps = 2 * (512 << 20) / (median(copy_ms) * 1e-3) / 1e9
     us, best = r["sweep_us"], r["sweep_best"]
-    kname = (f"segsum_kernel<{'double' if vs == 8 else 'float'},{S}>" if cfg[0] == 0 else
-             f"csrvec_kernel<{'double' if vs == 8 else 'float'},{cfg[4]}>")
+    tname = "double" if vs == 8 else "float"
+    kname = (f"csrvec_kernel<{tname},{cfg[4]}>" if cfg[0] == 1 else
+             f"segsum_kernel<{tname},{S}>" + (f" over {cfg[4]} column panels" if cfg[0] == 2
+                                              else ""))
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -236,11 +269,14 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": a.dtype,
-        "data": "synthetic (seeded generator), inputs resident in HBM, L2+MALL flushed per step",
+        "data": ("Matrix Market file" if a.mtx else "synthetic (seeded generator)") +
+                ", inputs resident in HBM, L2+MALL flushed per step",
         "config": {
             "workload": A.name, "m": A.m, "n": A.n, "nnz": A.nnz,
-            "effective_bytes": b, "kernel": "speculative segmented sum" if cfg[0] == 0 else
-            "csr-vector", "tile": tuner.class_name(cfg), "tile_source": r["cfg_source"],
+            "effective_bytes": b,
+            "kernel": {0: "speculative segmented sum", 1: "csr-vector",
+                       2: "speculative segmented sum over column panels"}[cfg[0]],
+            "tile": tuner.class_name(cfg), "tile_source": r["cfg_source"],
             "parallelism": "single GPU",
         },
         "pct_hbm_peak": round(100 * value / HBM_PEAK_GBPS, 2),
@@ -285,6 +321,10 @@ def main():
             "init_data_s": round(r["init_s"], 4),
         },
     }
+    if not a.no_vendor:
+        v = run_vendor(A, x, dtype, a.flush_mb << 20, r["y"])
+        v["bhs_speedup_vs_best"] = round(v["best_cold_us"] / (med_total * 1e3), 3)
+        out["vendor"] = v
     if not a.no_batched:
         out["batched"] = run_batched()
     if not a.no_cpu:

@@ -196,6 +196,8 @@ def host_lib():
     L.bhs_gen_block3.restype = P
     L.bhs_gen_banded.argtypes = [C.c_ulonglong, C.c_int, C.c_int, C.c_int]
     L.bhs_gen_banded.restype = P
+    L.bhs_mm_read.argtypes = [C.c_char_p, C.c_int, C.POINTER(P)]
+    L.bhs_mm_read.restype = C.c_int
     _host = L
     return L
 

@@ -53,11 +53,24 @@ def build_hip(jobs: int = 4) -> str:
     return out
 
 
+def build_vendor() -> str:
+    """libbhs_vendor.so: the rocSPARSE comparison column (its own library, so the
+    drop-in library does not depend on rocSPARSE)."""
+    src = os.path.join(CSRC, "bhs_vendor.hip")
+    obj = os.path.join(CSRC, "bhs_vendor.o")
+    _run([HIPCC, *HIP_FLAGS, "-c", src, "-o", obj])
+    out = os.path.join(PKG, "libbhs_vendor.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", obj, "-lrocsparse", "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_host() -> str:
     out = os.path.join(PKG, "libbhs_host.so")
     tmp = out + ".tmp"
     _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-Wall",
-          os.path.join(CSRC, "matgen.cpp"), "-o", tmp])
+          os.path.join(CSRC, "matgen.cpp"), os.path.join(CSRC, "mmio.cpp"), "-o", tmp])
     os.replace(tmp, out)
     return out
 
@@ -74,11 +87,10 @@ def build_example() -> str:
 
 
 def build_all() -> None:
-    with ThreadPoolExecutor(max_workers=2) as ex:
-        fh = ex.submit(build_hip)
-        fo = ex.submit(build_host)
-        fh.result()
-        fo.result()
+    with ThreadPoolExecutor(max_workers=3) as ex:
+        futs = [ex.submit(build_hip), ex.submit(build_host), ex.submit(build_vendor)]
+        for f in futs:
+            f.result()
     build_example()
 
 

@@ -67,8 +67,15 @@ struct Impl {
     // the fixed default tile (the tuner's "no tuner" baseline class)
     bhs_config cfg{BHS_VARIANT_SEGSUM, 8, 4, 4, 0};
 
-    // analysis for the current tile shape
-    int an_P = 0, an_TPW = 0;
+    // column-panel copy of A (BHS_VARIANT_SEGSUM_PANELS): one stacked row per
+    // nonempty (panel, row) pair, panel-major; pan_idx[p*m+i] = its stacked row
+    // or -1; pan_n = 0 when not built
+    int pan_n = 0, pan_rows = 0;
+    int *d_prow_ptr = nullptr, *d_pcol = nullptr, *d_pidx = nullptr;
+    T *d_pval = nullptr, *d_ystack = nullptr;
+
+    // analysis for the current tile shape and layout (an_layout = panels, 0 = A)
+    int an_P = 0, an_TPW = 0, an_layout = -1;
     int NP = 0, NW = 0, lead_empty = 0;
     int *d_tile_row = nullptr;
     int tile_cap = 0;
@@ -173,19 +180,34 @@ void release_data(Impl<T> *im)
     free_ptr(im->d_sync_row);
     free_ptr(im->d_sync_val);
     free_ptr(im->d_floor);
+    free_ptr(im->d_prow_ptr);
+    free_ptr(im->d_pcol);
+    free_ptr(im->d_pval);
+    free_ptr(im->d_ystack);
+    free_ptr(im->d_pidx);
+    im->pan_n = im->pan_rows = 0;
     im->floor_cap = 0;
     im->tile_cap = im->sync_cap = 0;
     im->an_P = im->an_TPW = 0;
+    im->an_layout = -1;
     im->NP = im->NW = im->lead_empty = 0;
     im->h_y = nullptr;
     im->data = false;
     im->m = im->n = im->nnz = 0;
 }
 
+inline bool is_segsum(int variant)
+{
+    return variant == BHS_VARIANT_SEGSUM || variant == BHS_VARIANT_SEGSUM_PANELS;
+}
+
 int check_config(const bhs_config &c)
 {
-    if (c.variant != BHS_VARIANT_SEGSUM && c.variant != BHS_VARIANT_CSRVEC)
+    if (c.variant != BHS_VARIANT_SEGSUM && c.variant != BHS_VARIANT_CSRVEC &&
+        c.variant != BHS_VARIANT_SEGSUM_PANELS)
         return set_error(BHSPARSE_INVALID_ARGUMENT, "unknown variant");
+    if (c.variant == BHS_VARIANT_SEGSUM_PANELS && c.tpv != 2 && c.tpv != 4 && c.tpv != 8)
+        return set_error(BHSPARSE_INVALID_ARGUMENT, "column panels (tpv) must be 2, 4 or 8");
     const int S = c.items_per_lane;
     if (S != 4 && S != 8 && S != 16 && S != 32)
         return set_error(BHSPARSE_INVALID_ARGUMENT, "items_per_lane must be 4, 8, 16 or 32");
@@ -199,13 +221,109 @@ int check_config(const bhs_config &c)
     return BHSPARSE_SUCCESS;
 }
 
-// Partition for the configured tile size and sizes of the per-wave buffers.
+// The matrix the segmented sum runs over: A itself, or its column-panel copy.
+template <typename T>
+struct Layout {
+    int rows;
+    const int *row_ptr, *col;
+    const T *val;
+    T *y;
+};
+
+template <typename T>
+Layout<T> active_layout(Impl<T> *im)
+{
+    if (im->cfg.variant == BHS_VARIANT_SEGSUM_PANELS)
+        return {im->pan_rows, im->d_prow_ptr, im->d_pcol, im->d_pval, im->d_ystack};
+    return {im->m, im->d_row_ptr, im->d_col, im->d_val, im->d_y};
+}
+
+// Column-panel copy of A: a host pass over the device copy (counting sort by
+// (panel, row)).  Only nonempty (panel, row) pairs become stacked rows, so the
+// stacked matrix has no empty rows; columns stay sorted inside every one.
+template <typename T>
+int build_panels(Impl<T> *im, int npan)
+{
+    if (im->pan_n == npan) return BHSPARSE_SUCCESS;
+    free_ptr(im->d_prow_ptr);
+    free_ptr(im->d_pcol);
+    free_ptr(im->d_pval);
+    free_ptr(im->d_ystack);
+    free_ptr(im->d_pidx);
+    im->pan_n = im->pan_rows = 0;
+    const int m = im->m, n = im->n, nnz = im->nnz;
+    if ((long long)npan * m > 0x7fffffffLL)
+        return set_error(BHSPARSE_INVALID_ARGUMENT, "too many rows for column panels");
+    std::vector<int> rp((size_t)m + 1), col((size_t)nnz);
+    std::vector<T> val((size_t)nnz);
+    BHS_HIP(hipMemcpyAsync(rp.data(), im->d_row_ptr, sizeof(int) * ((size_t)m + 1),
+                           hipMemcpyDeviceToHost, im->stream));
+    if (nnz > 0) {
+        BHS_HIP(hipMemcpyAsync(col.data(), im->d_col, sizeof(int) * (size_t)nnz,
+                               hipMemcpyDeviceToHost, im->stream));
+        BHS_HIP(hipMemcpyAsync(val.data(), im->d_val, sizeof(T) * (size_t)nnz,
+                               hipMemcpyDeviceToHost, im->stream));
+    }
+    BHS_HIP(hipStreamSynchronize(im->stream));
+    const int pw = (n + npan - 1) / npan;
+    const size_t PM = (size_t)npan * (size_t)m;
+    std::vector<int> cnt(PM, 0);   // entries of row i in panel p, at p*m+i
+    for (int i = 0; i < m; ++i)
+        for (int j = rp[i]; j < rp[i + 1]; ++j) ++cnt[(size_t)(col[j] / pw) * m + i];
+    std::vector<int> idx(PM), prp(1, 0);
+    prp.reserve(PM + 1);
+    for (size_t q = 0; q < PM; ++q) {
+        if (cnt[q] > 0) {
+            idx[q] = (int)prp.size() - 1;
+            prp.push_back(prp.back() + cnt[q]);
+        } else {
+            idx[q] = -1;
+        }
+    }
+    const int M = (int)prp.size() - 1;
+    std::vector<int> pos(prp.begin(), prp.end() - 1), pcol((size_t)nnz);
+    std::vector<T> pval((size_t)nnz);
+    for (int i = 0; i < m; ++i)
+        for (int j = rp[i]; j < rp[i + 1]; ++j) {
+            const int dst = pos[(size_t)idx[(size_t)(col[j] / pw) * m + i]]++;
+            pcol[(size_t)dst] = col[j];
+            pval[(size_t)dst] = val[j];
+        }
+    BHS_HIP(hipMalloc(&im->d_prow_ptr, sizeof(int) * ((size_t)M + 1)));
+    BHS_HIP(hipMalloc(&im->d_pcol, sizeof(int) * (size_t)im->nnz_pad));
+    BHS_HIP(hipMalloc(&im->d_pval, sizeof(T) * (size_t)im->nnz_pad));
+    BHS_HIP(hipMalloc(&im->d_ystack, sizeof(T) * (size_t)std::max(M, 1)));
+    BHS_HIP(hipMalloc(&im->d_pidx, sizeof(int) * std::max<size_t>(PM, 1)));
+    BHS_HIP(hipMemcpyAsync(im->d_prow_ptr, prp.data(), sizeof(int) * ((size_t)M + 1),
+                           hipMemcpyHostToDevice, im->stream));
+    BHS_HIP(hipMemcpyAsync(im->d_pidx, idx.data(), sizeof(int) * PM, hipMemcpyHostToDevice,
+                           im->stream));
+    BHS_HIP(hipMemsetAsync(im->d_pcol, 0, sizeof(int) * (size_t)im->nnz_pad, im->stream));
+    BHS_HIP(hipMemsetAsync(im->d_pval, 0, sizeof(T) * (size_t)im->nnz_pad, im->stream));
+    if (nnz > 0) {
+        BHS_HIP(hipMemcpyAsync(im->d_pcol, pcol.data(), sizeof(int) * (size_t)nnz,
+                               hipMemcpyHostToDevice, im->stream));
+        BHS_HIP(hipMemcpyAsync(im->d_pval, pval.data(), sizeof(T) * (size_t)nnz,
+                               hipMemcpyHostToDevice, im->stream));
+    }
+    BHS_HIP(hipStreamSynchronize(im->stream));
+    im->pan_n = npan;
+    im->pan_rows = M;
+    return BHSPARSE_SUCCESS;
+}
+
+// Partition for the configured tile size and layout, and sizes of the
+// per-wave buffers.
 template <typename T>
 int ensure_analysis(Impl<T> *im)
 {
     const int P = 64 * im->cfg.items_per_lane;
     const int TPW = im->cfg.tiles_per_wave;
-    if (im->an_P == P && im->an_TPW == TPW) return BHSPARSE_SUCCESS;
+    const int layout = im->cfg.variant == BHS_VARIANT_SEGSUM_PANELS ? im->cfg.tpv : 0;
+    if (im->an_P == P && im->an_TPW == TPW && im->an_layout == layout) return BHSPARSE_SUCCESS;
+    if (layout > 0 && im->m > 0)
+        if (int rc = build_panels(im, layout)) return rc;
+    const Layout<T> L = active_layout(im);
     const long long NP = ((long long)im->nnz + P - 1) / P;
     const long long NW = (NP + TPW - 1) / TPW;
     if (NP + 1 > im->tile_cap) {
@@ -222,17 +340,19 @@ int ensure_analysis(Impl<T> *im)
     }
     const int threads = 256;
     const int blocks = (int)((NP + 1 + threads - 1) / threads);
-    hipLaunchKernelGGL(bhs::partition_kernel, dim3(blocks), dim3(threads), 0, im->stream,
-                       im->d_row_ptr, im->m, im->nnz, P, (int)NP, im->d_tile_row);
+    if (L.rows > 0)
+        hipLaunchKernelGGL(bhs::partition_kernel, dim3(blocks), dim3(threads), 0, im->stream,
+                           L.row_ptr, L.rows, im->nnz, P, (int)NP, im->d_tile_row);
     BHS_HIP(hipGetLastError());
     int first = 0;
     BHS_HIP(hipMemcpyAsync(&first, im->d_tile_row, sizeof(int), hipMemcpyDeviceToHost, im->stream));
     BHS_HIP(hipStreamSynchronize(im->stream));
     im->NP = (int)NP;
     im->NW = (int)NW;
-    im->lead_empty = first;
+    im->lead_empty = L.rows > 0 ? first : 0;
     im->an_P = P;
     im->an_TPW = TPW;
+    im->an_layout = layout;
     return BHSPARSE_SUCCESS;
 }
 
@@ -306,8 +426,9 @@ int set_x(bhs_handle_t h, const T *x)
 template <typename T, int S, bool STATS>
 void launch_segsum_s(Impl<T> *im, T alpha, unsigned long long *stats)
 {
+    const Layout<T> L = active_layout(im);
     bhs::SegsumParams<T> q;
-    q.m = im->m;
+    q.m = L.rows;
     q.nnz = im->nnz;
     q.NP = im->NP;
     q.TPW = im->cfg.tiles_per_wave;
@@ -319,8 +440,8 @@ void launch_segsum_s(Impl<T> *im, T alpha, unsigned long long *stats)
     const int blocks = (im->NW + q.wpb - 1) / q.wpb;
     const size_t lds = (size_t)q.wpb * q.wave_lds_bytes;
     hipLaunchKernelGGL((bhs::segsum_kernel<T, S, STATS>), dim3(blocks), dim3(64 * q.wpb), lds,
-                       im->stream, im->d_row_ptr, im->d_col, im->d_val, im->d_x, im->d_y,
-                       im->d_tile_row, im->d_sync_row, im->d_sync_val, stats, q);
+                       im->stream, L.row_ptr, L.col, L.val, im->d_x, L.y, im->d_tile_row,
+                       im->d_sync_row, im->d_sync_val, stats, q);
 }
 
 template <typename T, bool STATS>
@@ -341,7 +462,17 @@ void launch_calibrate(Impl<T> *im)
     const int threads = 256;
     hipLaunchKernelGGL(bhs::calibrate_kernel<T>, dim3((im->NW + threads - 1) / threads),
                        dim3(threads), 0, im->stream, im->d_sync_row, im->d_sync_val, im->NW,
-                       im->d_y);
+                       active_layout(im).y);
+}
+
+template <typename T>
+void launch_combine(Impl<T> *im)
+{
+    if (im->cfg.variant != BHS_VARIANT_SEGSUM_PANELS) return;
+    const int threads = 256;
+    const long long blocks = std::min<long long>(((long long)im->m + threads - 1) / threads, 8192);
+    hipLaunchKernelGGL(bhs::combine_panels_kernel<T>, dim3((unsigned)blocks), dim3(threads), 0,
+                       im->stream, im->d_ystack, im->d_pidx, im->m, im->pan_n, im->d_y);
 }
 
 // TPV rule of main.cu:238-287 / csr_vector_opencl.h:212-230 (by floor(nnz/m)).
@@ -429,6 +560,8 @@ int enqueue_spmv(Impl<T> *im, T alpha, hipEvent_t main_done = nullptr)
     BHS_HIP(hipGetLastError());
     if (main_done) BHS_HIP(hipEventRecord(main_done, im->stream));
     launch_calibrate(im);
+    BHS_HIP(hipGetLastError());
+    launch_combine(im);
     BHS_HIP(hipGetLastError());
     return BHSPARSE_SUCCESS;
 }
@@ -644,8 +777,8 @@ void launch_floor_s(Impl<T> *im)
 {
     const long long threads = (long long)im->NP * 64;
     hipLaunchKernelGGL((bhs::floor_kernel<T, S>), dim3((unsigned)((threads + 255) / 256)),
-                       dim3(256), 0, im->stream, im->d_col, im->d_val, im->d_x, im->NP,
-                       im->d_floor);
+                       dim3(256), 0, im->stream, active_layout(im).col, active_layout(im).val,
+                       im->d_x, im->NP, im->d_floor);
 }
 
 template <typename T>
@@ -718,7 +851,7 @@ int autotune(bhs_handle_t h, const bhs_config *cands, int n, int reps, long long
     int bi = -1;
     for (int i = 0; i < n; ++i) {
         us[i] = -1;
-        if (cands[i].variant == BHS_VARIANT_SEGSUM && segsum_lds_bytes<T>(cands[i]) > im->max_lds)
+        if (is_segsum(cands[i].variant) && segsum_lds_bytes<T>(cands[i]) > im->max_lds)
             continue;
         im->cfg = cands[i];
         if (int rc = bench<T>(h, T(1), 2, reps, flush_bytes, t.data(), nullptr)) {
@@ -762,7 +895,7 @@ int apply_tuner(bhs_handle_t h, bhs_mlp_t net, const bhs_config *classes, int K,
     io[3] = -1;
     for (int k = 0; k < K; ++k) {
         const bool ok = check_config(classes[k]) == BHSPARSE_SUCCESS &&
-                        !(classes[k].variant == BHS_VARIANT_SEGSUM &&
+                        !(is_segsum(classes[k].variant) &&
                           segsum_lds_bytes<T>(classes[k]) > im->max_lds);
         io[4 + k] = ok;
     }

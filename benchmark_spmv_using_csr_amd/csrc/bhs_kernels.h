@@ -92,18 +92,37 @@ __device__ __forceinline__ void tile_load(const int *__restrict__ col, const T *
     }
 }
 
+// Row boundaries of the first 128 rows of a tile starting at row r: lane l holds
+// row_ptr[r+l], row_ptr[r+l+1], row_ptr[r+64+l], row_ptr[r+64+l+1], with the
+// row index clamped to m-1 (every address in bounds; callers ignore rows past
+// the tile's last row).  Loaded one tile ahead of its use.
+struct RowChunk {
+    int a0, b0, a1, b1;
+};
+
+__device__ __forceinline__ RowChunk load_chunk(const int *__restrict__ row_ptr, int r, int lane,
+                                               int m)
+{
+    const int i0 = min(r + lane, m - 1);
+    const int i1 = min(r + kWave + lane, m - 1);
+    return RowChunk{row_ptr[i0], row_ptr[i0 + 1], row_ptr[i1], row_ptr[i1 + 1]};
+}
+
 // ---------------------------------------------------------------------------
 // Speculative segmented sum (kernels_cuda.h:7-269, wave64 form).
 //
 // Per tile [B, B+P) with r0 = tile_row[p], r1 = tile_row[p+1]:
 //  * products val*x[col] (kernels_cuda.h:150-162): coalesced loads, the x
 //    gathers all issued before any of them is waited on; the next tile's
-//    loads are in flight while this tile is reduced.
+//    col/val stream and the row_ptr values of its first 128 rows are in
+//    flight while this tile is reduced, so no tile waits on a row_ptr load
+//    unless it spans more than 128 rows.
 //  * fast track (r0 == r1, the tile lies inside one row, :79-112): lane sums,
 //    wave sum, added to the carried partial; nothing is stored until the row
 //    closes.
 //  * otherwise
-//    - bit flags (:114-148): lanes walk rows [r0, min(r1, m-1)]; a nonempty
+//    - bit flags (:114-148): each lane takes rows r0+l and r0+64+l (further
+//      rows, rare, in a loop) of [r0, min(r1, m-1)]; a nonempty
 //      row starting at offset o in [0, P) sets bit o%S of lane o/S's word
 //      (LDS atomic OR); an empty row marks the tile dirty and gets y = 0.
 //    - the products go through LDS so lane l owns items [lS, lS+S) in order
@@ -171,7 +190,7 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
     const int p_end = min(p_begin + mv.TPW, mv.NP);
     const int E0 = p_begin * P;
     const int r_first = tile_row[p_begin];
-    bool syncing = row_ptr[r_first] < E0;    // first row began in an earlier wave
+    bool syncing = false;                       // first row began in an earlier wave
     bool did_sync = false;
     bool open = false;                          // a row continues past the last tile
     T carry1 = T(0);                            // the open row's partial (KMAX == 1)
@@ -192,8 +211,11 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
     };
     unsigned long long n_fast = 0, n_dirty = 0;
 
+    // Row chunk of the next tile loaded one tile ahead; the next tile's stream
+    // is in flight while this one is reduced (the last tile reloads itself).
     Tile<T, S> cur;
     tile_load<T, S>(mv.col, mv.val, E0, lane, cur);
+    RowChunk rc = load_chunk(row_ptr, r_first, lane, mv.m);
     int r1 = r_first;
 
     for (int p = p_begin; p < p_end; ++p) {
@@ -201,30 +223,17 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
         const int r0 = r1;
         r1 = tile_row[p + 1];
         const bool general = r0 != r1;
-
-        // all gathers of this tile (first right-hand side) in flight at once
         T xv[S];
 #pragma unroll
         for (int kk = 0; kk < S; ++kk) xv[kk] = x[(long long)cur.c[kk] * ks];
-
-        // first chunk of the flag walk (rows r0 .. r0+63), loaded up front.
-        // Loads below are unconditional (clamped addresses, always in bounds):
-        // a load under a branch makes the compiler drain every outstanding
-        // load where the branch rejoins.
-        const int stop = (r1 == mv.m) ? mv.m - 1 : r1;
-        const bool have0 = general && (r0 + lane <= stop);
-        const int ri = min(r0 + lane, mv.m - 1);
-        const int rp0 = row_ptr[ri];
-        const int rpn0 = row_ptr[ri + 1];
-        const bool cont = read_lane(rp0, 0) < B;   // item 0 continues row r0 (lane 0 holds row r0)
-
-        // keep the prefetch behind the loads this tile waits for (the vector
-        // memory counter retires in order)
+        const RowChunk cc = rc;
+        rc = load_chunk(row_ptr, r1, lane, mv.m);
         __builtin_amdgcn_sched_barrier(0);
-        // next tile's stream in flight while this one is reduced (the last
-        // tile of the wave reloads itself instead of branching)
         Tile<T, S> nxt;
         tile_load<T, S>(mv.col, mv.val, (p + 1 < p_end) ? B + P : B, lane, nxt);
+
+        const bool cont = read_lane(cc.a0, 0) < B;   // item 0 continues row r0 (lane 0 holds row r0)
+        if (p == p_begin) syncing = cont;
         const bool tail = B + P > mv.nnz;
 
         if (!general) {                          // fast track
@@ -247,129 +256,143 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
             }
             open = true;
             if (STATS) ++n_fast;
-            cur = nxt;
-            continue;
-        }
-
-        // ---- bit flags (once for all right-hand sides) ----------------------
-        wave_sync();                             // previous tile's LDS reads are done
-        s_head[lane] = 0u;
-        wave_sync();
-        const bool ne0 = have0 && rpn0 > rp0;
-        bool lane_dirty = have0 && !ne0;
-        if (ne0) {
-            const int o = rp0 - B;
-            if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
-        } else if (have0) {
-            for (int rr = 0; rr < k; ++rr) y[(long long)(r0 + lane) * ks + rr] = T(0);
-        }
-        for (int r = r0 + kWave + lane; r <= stop; r += kWave) {   // rare: > 64 rows
-            const int rp = row_ptr[r], rpn = row_ptr[r + 1];
-            if (rpn > rp) {
-                const int o = rp - B;
+        } else {
+            // ---- bit flags (once for all right-hand sides) ------------------
+            const int stop = (r1 == mv.m) ? mv.m - 1 : r1;
+            const bool have0 = r0 + lane <= stop;
+            const bool have1 = r0 + kWave + lane <= stop;
+            const bool ne0 = have0 && cc.b0 > cc.a0;
+            const bool ne1 = have1 && cc.b1 > cc.a1;
+            bool lane_dirty = (have0 && !ne0) || (have1 && !ne1);
+            wave_sync();                         // previous tile's LDS reads are done
+            s_head[lane] = 0u;
+            wave_sync();
+            if (ne0) {
+                const int o = cc.a0 - B;
                 if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
-            } else {
-                lane_dirty = true;
-                for (int rr = 0; rr < k; ++rr) y[(long long)r * ks + rr] = T(0);
+            } else if (have0) {
+                for (int rr = 0; rr < k; ++rr) y[(long long)(r0 + lane) * ks + rr] = T(0);
             }
-        }
-        const bool dirty = ballot(lane_dirty) != 0ull;
-        wave_sync();
-        unsigned hb = s_head[lane];
-        if (lane == 0) hb |= 1u;
-
-        // ---- segment numbers ---------------------------------------------
-        const int nh = __popc(hb);
-        const int incl = wave_incl_scan(nh, lane);
-        const int seg_base = incl - nh;
-        const int K = read_lane(incl, kWave - 1);
-        // row_ptr[r1] sits in lane r1-r0 of the first chunk when r1 < r0+64
-        int rp_r1;
-        if (r1 - r0 < kWave) rp_r1 = read_lane(rp0, r1 - r0);
-        else rp_r1 = row_ptr[min(r1, mv.m)];
-        const bool partial_tail = (r1 < mv.m) && (rp_r1 < B + P);
-        const int kb = syncing ? 1 : 0;
-        const int ke = partial_tail ? K - 1 : K;
-        const uint64_t m0 = ballot(ne0);
-        const int rank0 = popc_below(m0);
-        if (STATS && dirty) ++n_dirty;
-
-        for (int rr = 0; rr < k; ++rr) {
-            if (rr > 0) {
-#pragma unroll
-                for (int kk = 0; kk < S; ++kk) xv[kk] = x[(long long)cur.c[kk] * ks + rr];
+            if (ne1) {
+                const int o = cc.a1 - B;
+                if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
+            } else if (have1) {
+                for (int rr = 0; rr < k; ++rr)
+                    y[(long long)(r0 + kWave + lane) * ks + rr] = T(0);
             }
-            T prod[S];
-#pragma unroll
-            for (int kk = 0; kk < S; ++kk) {
-                const T pk = cur.v[kk] * xv[kk];
-                prod[kk] = tail ? keep_or_zero(pk, tile_item<T, S>(B, lane, kk) < mv.nnz) : pk;
-            }
-            // products -> LDS in tile order, back lane-contiguous
-            wave_sync();
-#pragma unroll
-            for (int j = 0; j < S / G; ++j) {
-                T *dst = s_buf + j * (kWave * G) + lane * G;
-#pragma unroll
-                for (int g = 0; g < G; ++g) dst[g] = prod[j * G + g];
-            }
-            wave_sync();
-            T pc[S];
-#pragma unroll
-            for (int kk = 0; kk < S; ++kk) pc[kk] = s_buf[lane * S + kk];
-            wave_sync();
-            if (lane == 0 && cont) pc[0] = pc[0] + carry_get(rr);
-
-            // ---- lane-sequential segmented reduction ---------------------
-            T acc = T(0), lead = T(0);
-            bool seen = false;
-            int sidx = seg_base;
-#pragma unroll
-            for (int kk = 0; kk < S; ++kk) {
-                if ((hb >> kk) & 1u) {
-                    if (seen) { s_buf[sidx] = acc; ++sidx; }
-                    else { lead = acc; seen = true; }
-                    acc = T(0);
-                }
-                acc += pc[kk];
-            }
-
-            // ---- segments spanning lanes ---------------------------------
-            const uint64_t present = ballot(seen);  // bit 0 always set
-            const T I = wave_segscan(acc, present, lane);
-            const T Iprev = __shfl_up(I, 1, kWave);
-            if (seen && lane > 0) s_buf[seg_base - 1] = Iprev + lead;
-            if (lane == kWave - 1) s_buf[K - 1] = I;
-            wave_sync();
-
-            // ---- output ----------------------------------------------------
-            if (!dirty) {
-                for (int kk = kb + lane; kk < ke; kk += kWave)
-                    y[(long long)(r0 + kk) * ks + rr] = alpha * s_buf[kk];
-            } else {
-                if (ne0 && rank0 >= kb && rank0 < ke)
-                    y[(long long)(r0 + lane) * ks + rr] = alpha * s_buf[rank0];
-                int running = __popcll(m0);
-                for (int rb = r0 + kWave; rb <= stop; rb += kWave) {
-                    const int r = rb + lane;
-                    bool ne = false;
-                    if (r <= stop) ne = row_ptr[r + 1] > row_ptr[r];
-                    const uint64_t msk = ballot(ne);
-                    const int rank = running + popc_below(msk);
-                    if (ne && rank >= kb && rank < ke)
-                        y[(long long)r * ks + rr] = alpha * s_buf[rank];
-                    running += __popcll(msk);
+            for (int r = r0 + 2 * kWave + lane; r <= stop; r += kWave) {   // rare: > 128 rows
+                const int rp = row_ptr[r], rpn = row_ptr[r + 1];
+                if (rpn > rp) {
+                    const int o = rp - B;
+                    if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
+                } else {
+                    lane_dirty = true;
+                    for (int rr = 0; rr < k; ++rr) y[(long long)r * ks + rr] = T(0);
                 }
             }
-            if (syncing && lane == 0) mv.sync_val[(long long)w * KMAX + rr] = alpha * s_buf[0];
-            carry_set(rr, s_buf[K - 1]);
+            const bool dirty = ballot(lane_dirty) != 0ull;
+            wave_sync();
+            unsigned hb = s_head[lane];
+            if (lane == 0) hb |= 1u;
+
+            // ---- segment numbers -----------------------------------------
+            const int nh = __popc(hb);
+            const int incl = wave_incl_scan(nh, lane);
+            const int seg_base = incl - nh;
+            const int K = read_lane(incl, kWave - 1);
+            // row_ptr[r1] sits in the chunk when r1 < r0+128
+            int rp_r1;
+            if (r1 - r0 < kWave) rp_r1 = read_lane(cc.a0, r1 - r0);
+            else if (r1 - r0 < 2 * kWave) rp_r1 = read_lane(cc.a1, r1 - r0 - kWave);
+            else rp_r1 = row_ptr[min(r1, mv.m)];
+            const bool partial_tail = (r1 < mv.m) && (rp_r1 < B + P);
+            const int kb = syncing ? 1 : 0;
+            const int ke = partial_tail ? K - 1 : K;
+            const uint64_t m0 = ballot(ne0);
+            const uint64_t m1 = ballot(ne1);
+            const int rank0 = popc_below(m0);
+            const int rank1 = __popcll(m0) + popc_below(m1);
+            if (STATS && dirty) ++n_dirty;
+
+            for (int rr = 0; rr < k; ++rr) {
+                if (rr > 0) {
+#pragma unroll
+                    for (int kk = 0; kk < S; ++kk) xv[kk] = x[(long long)cur.c[kk] * ks + rr];
+                }
+                T prod[S];
+#pragma unroll
+                for (int kk = 0; kk < S; ++kk) {
+                    const T pk = cur.v[kk] * xv[kk];
+                    prod[kk] = tail ? keep_or_zero(pk, tile_item<T, S>(B, lane, kk) < mv.nnz) : pk;
+                }
+                // products -> LDS in tile order, back lane-contiguous
+                wave_sync();
+#pragma unroll
+                for (int j = 0; j < S / G; ++j) {
+                    T *dst = s_buf + j * (kWave * G) + lane * G;
+#pragma unroll
+                    for (int g = 0; g < G; ++g) dst[g] = prod[j * G + g];
+                }
+                wave_sync();
+                T pc[S];
+#pragma unroll
+                for (int kk = 0; kk < S; ++kk) pc[kk] = s_buf[lane * S + kk];
+                wave_sync();
+                if (lane == 0 && cont) pc[0] = pc[0] + carry_get(rr);
+
+                // ---- lane-sequential segmented reduction -----------------
+                T acc = T(0), lead = T(0);
+                bool seen = false;
+                int sidx = seg_base;
+#pragma unroll
+                for (int kk = 0; kk < S; ++kk) {
+                    if ((hb >> kk) & 1u) {
+                        if (seen) { s_buf[sidx] = acc; ++sidx; }
+                        else { lead = acc; seen = true; }
+                        acc = T(0);
+                    }
+                    acc += pc[kk];
+                }
+
+                // ---- segments spanning lanes -----------------------------
+                const uint64_t present = ballot(seen);  // bit 0 always set
+                const T I = wave_segscan(acc, present, lane);
+                const T Iprev = __shfl_up(I, 1, kWave);
+                if (seen && lane > 0) s_buf[seg_base - 1] = Iprev + lead;
+                if (lane == kWave - 1) s_buf[K - 1] = I;
+                wave_sync();
+
+                // ---- output ------------------------------------------------
+                if (!dirty) {
+                    for (int kk = kb + lane; kk < ke; kk += kWave)
+                        y[(long long)(r0 + kk) * ks + rr] = alpha * s_buf[kk];
+                } else {
+                    if (ne0 && rank0 >= kb && rank0 < ke)
+                        y[(long long)(r0 + lane) * ks + rr] = alpha * s_buf[rank0];
+                    if (ne1 && rank1 >= kb && rank1 < ke)
+                        y[(long long)(r0 + kWave + lane) * ks + rr] = alpha * s_buf[rank1];
+                    int running = __popcll(m0) + __popcll(m1);
+                    for (int rb = r0 + 2 * kWave; rb <= stop; rb += kWave) {   // rare
+                        const int r = rb + lane;
+                        bool ne = false;
+                        if (r <= stop) ne = row_ptr[r + 1] > row_ptr[r];
+                        const uint64_t msk = ballot(ne);
+                        const int rank = running + popc_below(msk);
+                        if (ne && rank >= kb && rank < ke)
+                            y[(long long)r * ks + rr] = alpha * s_buf[rank];
+                        running += __popcll(msk);
+                    }
+                }
+                if (syncing && lane == 0) mv.sync_val[(long long)w * KMAX + rr] = alpha * s_buf[0];
+                carry_set(rr, s_buf[K - 1]);
+            }
+            if (syncing) {
+                if (lane == 0) mv.sync_row[w] = mv.key_base + r0 * ks;
+                syncing = false;
+                did_sync = true;
+            }
+            open = partial_tail;
         }
-        if (syncing) {
-            if (lane == 0) mv.sync_row[w] = mv.key_base + r0 * ks;
-            syncing = false;
-            did_sync = true;
-        }
-        open = partial_tail;
         cur = nxt;
     }
 
@@ -594,6 +617,26 @@ __global__ __launch_bounds__(256) void calibrate_batched_kernel(const int *__res
     kw = __builtin_amdgcn_readfirstlane(kw);
     for (int rr = 0; rr < kw; ++rr)
         calibrate_window<T, KMAX>(sync_row, sync_val, sync_k, NW, base, rr, lane, y);
+}
+
+// y[i] = sum over panels p (in order) of row i's panel partial: ystack[idx[p*m+i]],
+// +0 where row i has no entry in panel p.  The last step of the column-panel
+// variant; every partial already carries alpha.
+template <typename T>
+__global__ __launch_bounds__(256) void combine_panels_kernel(const T *__restrict__ ystack,
+                                                            const int *__restrict__ idx, int m,
+                                                            int npan, T *__restrict__ y)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+         i += (long long)gridDim.x * blockDim.x) {
+        int k = idx[i];
+        T s = k >= 0 ? ystack[k] : T(0);
+        for (int p = 1; p < npan; ++p) {
+            k = idx[(long long)p * m + i];
+            s += k >= 0 ? ystack[k] : T(0);
+        }
+        y[i] = s;
+    }
 }
 
 template <typename T>

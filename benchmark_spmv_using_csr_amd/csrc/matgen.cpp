@@ -21,14 +21,7 @@
 #include <omp.h>
 #endif
 
-extern "C" {
-typedef struct {
-    int m, n, nnz;
-    int *row_ptr;
-    int *col;
-    double *val;
-} bhs_csr;
-}
+#include "../../include/bhs_mmio.h"   // bhs_csr
 
 namespace {
 

@@ -11,7 +11,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from ._lib import host_lib
+from ._lib import Csr, host_lib
 
 
 @dataclass
@@ -51,6 +51,40 @@ def _take(ptr, name) -> CSR:
     val = np.ctypeslib.as_array(A.val, shape=(max(nnz, 1),))[:nnz].copy()
     host_lib().bhs_csr_free(ptr)
     return CSR(m, n, rp, col, val, name)
+
+
+MM_SORT_COLUMNS = 1
+MM_MERGE_DUPLICATES = 2
+
+
+def read_mtx(path, sort=True, merge=True) -> CSR:
+    """Matrix Market coordinate file -> CSR (bhs_mm_read, include/bhs_mmio.h):
+    pattern -> 1.0, symmetric/hermitian mirrored, skew-symmetric mirrored with
+    the sign flipped.  sort / merge: sort every row by column / add duplicate
+    entries; both False keeps the reference's layout (main.cu:581-733)."""
+    from ._lib import BhsError
+    flags = (MM_SORT_COLUMNS if sort else 0) | (MM_MERGE_DUPLICATES if merge else 0)
+    ptr = C.POINTER(Csr)()
+    rc = host_lib().bhs_mm_read(str(path).encode(), flags, C.byref(ptr))
+    if rc != 0:
+        raise BhsError(rc, "bhs_mm_read", str(path))
+    return _take(ptr, str(path))
+
+
+def write_mtx(A: CSR, path, field="real", symmetry="general") -> None:
+    """Writes A as a Matrix Market coordinate file (every stored entry, 1-based;
+    with symmetry != general the caller passes the lower triangle)."""
+    with open(path, "w") as f:
+        f.write(f"%%MatrixMarket matrix coordinate {field} {symmetry}\n")
+        f.write(f"{A.m} {A.n} {A.nnz}\n")
+        rows = np.repeat(np.arange(A.m), np.diff(A.row_ptr))
+        for r, c, v in zip(rows.tolist(), A.col.tolist(), A.val.tolist()):
+            if field == "pattern":
+                f.write(f"{r + 1} {c + 1}\n")
+            elif field == "integer":
+                f.write(f"{r + 1} {c + 1} {int(v)}\n")
+            else:
+                f.write(f"{r + 1} {c + 1} {v!r}\n")     # shortest repr: round-trips
 
 
 def vector(seed: int, n: int) -> np.ndarray:

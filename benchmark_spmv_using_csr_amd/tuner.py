@@ -4,8 +4,8 @@ The reference fixes its tile shape at compile time (THREADBUNCH / SEG_H /
 STEP, spmv_cuda/common.h:39-48) and picks the csr-vector width by nnz/m
 (main.cu:238-287).  Here a small MLP picks, per matrix, one of CLASSES -- the
 segmented sum at a tile shape (items per lane x waves per block x tiles per
-wave) or csr-vector at a width -- from 8 row-length statistics that the GPU
-reduces from row_ptr/col.  The network predicts log2(time / best time) for
+wave), the same over a column-panel copy of A, or csr-vector at a width --
+from 8 row-length statistics that the GPU reduces from row_ptr/col.  The network predicts log2(time / best time) for
 every class; the tuner takes the argmin.  It is trained on timing sweeps this
 module generates (the exhaustive autotuner, bhs_*_autotune, which also judges
 the tuner's regret), by the library's own HIP forward/backward/Adam kernels.
@@ -33,7 +33,9 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 DATA_DIR = os.path.join(PKG, "data")
 DEFAULT_NET = os.path.join(DATA_DIR, "tuner_net.json")
 
-# (variant, items per lane, waves per block, tiles per wave, csr-vector width)
+# (variant, items per lane, waves per block, tiles per wave, csr-vector width or
+# column panels): the segmented sum, csr-vector, and the segmented sum over a
+# column-panel copy of A
 CLASSES = [(0, S, wpb, T, 0) for S in (4, 8, 16) for wpb in (1, 4) for T in (1, 4, 8, 16)] + \
           [(1, 8, 4, 1, tpv) for tpv in (2, 4, 8, 16, 32, 64)]
 K = len(CLASSES)
@@ -48,7 +50,11 @@ FAMILIES = ("uniform rows", "power law", "banded", "3x3 blocks")
 
 def class_name(c):
     v, S, wpb, T, tpv = c
-    return f"csr-vector tpv={tpv}" if v == 1 else f"segsum S={S} wpb={wpb} T={T}"
+    if v == 1:
+        return f"csr-vector tpv={tpv}"
+    if v == 2:
+        return f"segsum {tpv} column panels S={S} wpb={wpb} T={T}"
+    return f"segsum S={S} wpb={wpb} T={T}"
 
 
 def transform(raw: np.ndarray) -> np.ndarray:

@@ -35,6 +35,14 @@ extern "C" {
 /* Kernel variants a handle can run (bhs_*_set_config). */
 #define BHS_VARIANT_SEGSUM  0   /* speculative segmented sum (kernels_cuda.h:7-269) */
 #define BHS_VARIANT_CSRVEC  1   /* csr-vector (kernels_cuda.h:272-342, CUSP v0.4.0) */
+/* The segmented sum over a column-panel copy of A (no reference counterpart):
+ * row i's entries in column panel p = [p*ceil(n/tpv), (p+1)*ceil(n/tpv)) form
+ * one row of a stacked matrix (panel-major, nonempty (panel, row) pairs only),
+ * so the x gathers of consecutive tiles stay in one panel of x (one L2's worth
+ * on a random-column matrix); a last kernel adds the tpv panel partials of
+ * every row in panel order.  tpv = panels (2, 4 or 8).  Built once per panel
+ * count from the device copy of A (host pass, O(nnz)) and kept with the handle. */
+#define BHS_VARIANT_SEGSUM_PANELS 2
 
 typedef struct bhs_handle_opaque *bhs_handle_t;
 /* The tuner's network (a device-resident MLP); see "learned tuner" below. */
@@ -52,8 +60,10 @@ typedef struct {
 } bhs_spmv_stats;
 
 /* Tile shape / variant.  S = items per lane (4, 8, 16, 32), waves_per_block
- * in [1, 8], tiles_per_wave >= 1; tpv (csr-vector lanes per row) in
- * {0 = rule of main.cu:238-287 extended to 64, 2, 4, 8, 16, 32, 64}. */
+ * in [1, 8], tiles_per_wave >= 1; tpv = csr-vector lanes per row in
+ * {0 = rule of main.cu:238-287 extended to 64, 2, 4, 8, 16, 32, 64} for
+ * BHS_VARIANT_CSRVEC, column panels in {2, 4, 8} for BHS_VARIANT_SEGSUM_PANELS,
+ * ignored for BHS_VARIANT_SEGSUM. */
 typedef struct {
     int variant;
     int items_per_lane;

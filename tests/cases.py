@@ -110,3 +110,23 @@ def adversarial(P, T, seed=0, integer=False):
     pl[rng.random(1500) < 0.1] = 0
     add("power-law with 10% empty", pl, n=5000)
     return cs
+
+
+def stack_panels(c, npan):
+    """The column-panel copy of a case (the layout of BHS_VARIANT_SEGSUM_PANELS):
+    one stacked row per nonempty (panel, row) pair, panel-major, holding row
+    i's entries whose column lies in panel p = col // ceil(n / npan), columns in
+    order.  Returns the stacked case and idx (npan x m, -1 where empty)."""
+    pw = (c.n + npan - 1) // npan
+    rows = np.repeat(np.arange(c.m, dtype=np.int64), np.diff(c.row_ptr))
+    key = (c.col.astype(np.int64) // pw) * c.m + rows
+    order = np.argsort(key, kind="stable")
+    cnt = np.bincount(key, minlength=npan * c.m)
+    nz = np.flatnonzero(cnt)
+    idx = np.full(npan * c.m, -1, np.int64)
+    idx[nz] = np.arange(len(nz))
+    rp = np.zeros(len(nz) + 1, np.int64)
+    rp[1:] = np.cumsum(cnt[nz])
+    st = Case(c.name + f" panels={npan}", len(nz), c.n, rp.astype(np.int32), c.col[order],
+              c.val[order], c.x)
+    return st, idx.reshape(npan, c.m)

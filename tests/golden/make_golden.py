@@ -12,6 +12,12 @@ Every expected output here is computed without the oracle:
                        empty row); y = dense int64 product.
 For the last three the tile partition tile_row[p] = max{r : row_ptr[r] <= min(pP, nnz)}
 is also stored for P in {256, 512, 1024}, found by a linear scan.
+  * mtx/*.mtx       -- Matrix Market files (general / symmetric / hermitian /
+                       skew-symmetric; real / integer / pattern; comments, blank
+                       lines, unsorted rows, duplicates) written from explicit
+                       entry lists, with the dense matrix those entries define
+                       (mirroring rules applied here, not by the reader) in
+                       mtx/expected.json, and files the reader must reject.
 All values are small integers, so every summation order gives the exact y in
 both float64 and float32.
 """
@@ -121,5 +127,65 @@ def main():
             "5000 x 64, 70% empty rows; y = dense int64 product")
 
 
+def write_mtx_fixtures():
+    d = os.path.join(HERE, "mtx")
+    os.makedirs(d, exist_ok=True)
+    expected = {}
+
+    def emit(name, m, n, field, sym, entries, extra_lines=()):
+        with open(os.path.join(d, name), "w") as f:
+            f.write(f"%%MatrixMarket matrix coordinate {field} {sym}\n")
+            f.write("% written by make_golden.py\n%\n")
+            f.write(f"{m} {n} {len(entries)}\n")
+            for k, (i, j, v) in enumerate(entries):
+                if k in extra_lines:
+                    f.write("\n% a comment between entries\n")
+                f.write(f"{i} {j}\n" if field == "pattern" else
+                        f"{i} {j} {int(v) if field == 'integer' else v!r}\n")
+        D = np.zeros((m, n))
+        for i, j, v in entries:
+            v = 1.0 if field == "pattern" else float(v)
+            D[i - 1, j - 1] += v
+            if sym in ("symmetric", "hermitian") and i != j:
+                D[j - 1, i - 1] += v
+            if sym == "skew-symmetric":
+                D[j - 1, i - 1] -= v
+        expected[name] = {"m": m, "n": n, "dense": D.tolist(),
+                          "entries_in_file": len(entries)}
+
+    emit("general_real.mtx", 4, 5, "real", "general",
+         [(1, 3, 0.5), (1, 1, -2.0), (3, 5, 1.25), (2, 2, 3.0), (3, 1, 4.5), (1, 3, 0.25),
+          (4, 4, -1.0)], extra_lines=(3,))
+    emit("symmetric_integer.mtx", 5, 5, "integer", "symmetric",
+         [(1, 1, 4), (2, 1, -1), (3, 2, -1), (3, 3, 4), (5, 1, 7), (5, 4, 2), (4, 4, 1)])
+    emit("hermitian_real.mtx", 3, 3, "real", "hermitian",
+         [(1, 1, 2.0), (2, 1, 0.5), (3, 2, -0.75)])
+    emit("pattern_general.mtx", 3, 4, "pattern", "general",
+         [(1, 4, 0), (3, 1, 0), (1, 2, 0), (2, 3, 0)])
+    emit("skew_real.mtx", 4, 4, "real", "skew-symmetric",
+         [(2, 1, 1.5), (3, 1, -2.0), (4, 3, 0.5)])
+    emit("empty_rows.mtx", 6, 6, "real", "general", [(2, 2, 1.0), (6, 1, -3.0)])
+    with open(os.path.join(d, "expected.json"), "w") as f:
+        json.dump(expected, f, indent=1)
+    # files the reader must reject, with the code it must return
+    bad = {
+        "complex.mtx": ("%%MatrixMarket matrix coordinate complex general\n2 2 1\n1 1 1.0 0.0\n",
+                        -1),
+        "array.mtx": ("%%MatrixMarket matrix array real general\n2 2\n1.0\n2.0\n3.0\n4.0\n",
+                      -3),
+        "bad_index.mtx": ("%%MatrixMarket matrix coordinate real general\n2 2 1\n3 1 1.0\n", -4),
+        "truncated.mtx": ("%%MatrixMarket matrix coordinate real general\n2 2 3\n1 1 1.0\n", -4),
+        "no_banner.mtx": ("2 2 1\n1 1 1.0\n", -3),
+        "skew_diagonal.mtx": ("%%MatrixMarket matrix coordinate real skew-symmetric\n2 2 1\n"
+                              "1 1 1.0\n", -4),
+    }
+    for name, (txt, code) in bad.items():
+        with open(os.path.join(d, name), "w") as f:
+            f.write(txt)
+    with open(os.path.join(d, "reject.json"), "w") as f:
+        json.dump({k: v[1] for k, v in bad.items()}, f, indent=1)
+
+
 if __name__ == "__main__":
     main()
+    write_mtx_fixtures()

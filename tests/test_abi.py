@@ -1,5 +1,5 @@
-"""The C ABI library loads, exports every symbol of include/bhsparse_spmv.h,
-and its host-side argument checks behave -- without touching a GPU."""
+"""The C ABI libraries load, export every symbol their headers in include/
+declare, and the host-side argument checks behave -- without touching a GPU."""
 import ctypes as C
 import os
 import re
@@ -11,9 +11,22 @@ from benchmark_spmv_using_csr_amd import _lib
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared():
-    txt = open(os.path.join(ROOT, "include", "bhsparse_spmv.h")).read()
+def _declared(header="bhsparse_spmv.h"):
+    txt = open(os.path.join(ROOT, "include", header)).read()
     return sorted(set(re.findall(r"\b(bhs_\w+)\s*\(", txt)))
+
+
+def test_every_header_is_covered():
+    assert sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")) == \
+        ["bhs_mmio.h", "bhs_vendor.h", "bhsparse_spmv.h"]
+
+
+def test_host_library_exports_the_reader():
+    L = _lib.host_lib()
+    names = _declared("bhs_mmio.h")
+    assert names == ["bhs_csr_free", "bhs_mm_read"]
+    for name in names:
+        assert hasattr(L, name), name
 
 
 def test_header_and_binding_agree():
@@ -24,6 +37,16 @@ def test_library_exports_every_declared_symbol():
     L = _lib.spmv_lib()
     for name in _declared():
         assert hasattr(L, name), name
+
+
+def test_vendor_library_exports_every_declared_symbol():
+    from benchmark_spmv_using_csr_amd import vendor
+    L = vendor.vendor_lib()
+    names = _declared("bhs_vendor.h")
+    assert names == ["bhs_vendor_d_csrmv_bench", "bhs_vendor_s_csrmv_bench", "bhs_vendor_version"]
+    for name in names:
+        assert hasattr(L, name), name
+    assert vendor.version().startswith("4.")
 
 
 def test_error_strings():
@@ -49,6 +72,9 @@ def test_host_side_argument_checks():
     assert L.bhs_d_set_config(h, C.byref(bad)) == _lib.BHSPARSE_INVALID_ARGUMENT
     bad = _lib.Config(0, 6, 4, 3, 0)
     assert L.bhs_d_set_config(h, C.byref(bad)) == _lib.BHSPARSE_INVALID_ARGUMENT
+    bad = _lib.Config(2, 8, 4, 3, 3)        # column panels must be 2, 4 or 8
+    assert L.bhs_d_set_config(h, C.byref(bad)) == _lib.BHSPARSE_INVALID_ARGUMENT
+    assert L.bhs_d_set_config(h, C.byref(_lib.Config(2, 8, 4, 3, 4))) == 0
     good = _lib.Config(0, 16, 2, 5, 0)
     assert L.bhs_d_set_config(h, C.byref(good)) == 0
     out = _lib.Config()

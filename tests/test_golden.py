@@ -82,7 +82,8 @@ def test_planted_oracle_mistake_breaks_a_pin(mistake):
 
 
 # ---- the HIP library against the fixtures (GPU) ----------------------------
-SEGSUM = [(0, 4, 1, 1, 0), (0, 8, 4, 3, 0), (0, 16, 2, 2, 0), (0, 32, 2, 1, 0), (0, 4, 4, 16, 0)]
+SEGSUM = [(0, 4, 1, 1, 0), (0, 8, 4, 3, 0), (0, 16, 2, 2, 0), (0, 32, 2, 1, 0), (0, 4, 4, 16, 0),
+          (2, 4, 1, 3, 2), (2, 8, 4, 2, 4), (2, 16, 2, 1, 8)]
 CSRVEC = [(1, 8, 4, 1, 0), (1, 8, 4, 1, 2), (1, 8, 4, 1, 64)]
 
 
@@ -100,7 +101,7 @@ def test_library_reproduces_golden(name, dtype):
                               d["val"].astype(dtype), d["x"].astype(dtype), y), "prepare_mem")
     for cfg in SEGSUM + CSRVEC:
         _lib.check(op.set_config(*cfg), "set_config")
-        if cfg[0] == 0:
+        if cfg[0] == 0:               # the plain layout's partition is the fixture's
             P = 64 * cfg[1]
             tr = op.partition()
             if str(P) in d.get("tile_row", {}):
