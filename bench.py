@@ -47,9 +47,10 @@ def load_traffic(name, cfg):
     try:
         with open(p) as f:
             d = json.load(f)
-        if tuple(d.get("config", ())) != tuple(cfg):
-            return None
-        return float(d["hbm_bytes_per_launch"])
+        for e in d.get("entries", [d]):
+            if tuple(e.get("config", ())) == tuple(cfg):
+                return float(e["hbm_bytes_per_launch"])
+        return None
     except Exception:
         return None
 

@@ -37,7 +37,10 @@ def main(argv=None):
                     choices=["power-law", "uniform", "laplace", "block3"])
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
     ap.add_argument("--config", default="", help="S,waves_per_block,tiles_per_wave")
-    ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=0,
+                    help="0 segmented sum, 1 csr-vector, 2 segmented sum over column panels")
+    ap.add_argument("--tpv", type=int, default=0,
+                    help="csr-vector lanes per row (variant 1) or column panels (variant 2)")
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--flush-mb", type=int, default=1024)
     ap.add_argument("--copy-mb", type=int, default=256)
@@ -48,7 +51,7 @@ def main(argv=None):
     _lib.check(op.init_platform(0), "init_platform")
     if a.config:
         S, wpb, tpw = (int(v) for v in a.config.split(","))
-        _lib.check(op.set_config(a.variant, S, wpb, tpw, 0), "set_config")
+        _lib.check(op.set_config(a.variant, S, wpb, tpw, a.tpv), "set_config")
     y = np.zeros(A.m, dtype)
     _lib.check(op.prepare_mem(A.m, A.n, A.nnz, A.row_ptr, A.col, A.val.astype(dtype),
                               x.astype(dtype), y), "prepare_mem")
