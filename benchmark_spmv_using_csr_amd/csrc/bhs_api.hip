@@ -6,6 +6,8 @@
 // and bhsparse_spmv_opencl (spmv_opencl_amd/bhsparse_spmv_opencl.h).  Unlike
 // them, every buffer is plain device memory (hipMalloc), nothing is computed
 // on the host per call, and a call never depends on y's previous contents.
+#include <hip/hip_ext.h>
+
 #include "bhs_kernels.h"
 #include "bhs_tuner.h"
 #include "../../include/bhsparse_spmv.h"
@@ -89,7 +91,9 @@ struct Impl {
     int *d_tuner_io = nullptr;                  // tuner inference inputs/outputs (ints)
     void *d_flush = nullptr;
     long long flush_cap = 0;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    // ev[0]/ev[2]: recorded around a whole call; ev[1]/ev[3]: attached to the
+    // main kernel's dispatch (hipExtLaunchKernelGGL), its exact duration
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     size_t max_lds = 65536;   // per-block LDS limit of the device
 };
 
@@ -424,7 +428,7 @@ int set_x(bhs_handle_t h, const T *x)
 }
 
 template <typename T, int S, bool STATS>
-void launch_segsum_s(Impl<T> *im, T alpha, unsigned long long *stats)
+void launch_segsum_s(Impl<T> *im, T alpha, unsigned long long *stats, hipEvent_t e0, hipEvent_t e1)
 {
     const Layout<T> L = active_layout(im);
     bhs::SegsumParams<T> q;
@@ -439,19 +443,26 @@ void launch_segsum_s(Impl<T> *im, T alpha, unsigned long long *stats)
     q.alpha = alpha;
     const int blocks = (im->NW + q.wpb - 1) / q.wpb;
     const size_t lds = (size_t)q.wpb * q.wave_lds_bytes;
-    hipLaunchKernelGGL((bhs::segsum_kernel<T, S, STATS>), dim3(blocks), dim3(64 * q.wpb), lds,
-                       im->stream, L.row_ptr, L.col, L.val, im->d_x, L.y, im->d_tile_row,
-                       im->d_sync_row, im->d_sync_val, stats, q);
+    if (e0 && e1)
+        hipExtLaunchKernelGGL((bhs::segsum_kernel<T, S, STATS>), dim3(blocks), dim3(64 * q.wpb),
+                              (std::uint32_t)lds, im->stream, e0, e1, 0u, L.row_ptr, L.col, L.val,
+                              (const T *)im->d_x, L.y, (const int *)im->d_tile_row,
+                              im->d_sync_row, im->d_sync_val, stats, q);
+    else
+        hipLaunchKernelGGL((bhs::segsum_kernel<T, S, STATS>), dim3(blocks), dim3(64 * q.wpb), lds,
+                           im->stream, L.row_ptr, L.col, L.val, im->d_x, L.y, im->d_tile_row,
+                           im->d_sync_row, im->d_sync_val, stats, q);
 }
 
 template <typename T, bool STATS>
-void launch_segsum(Impl<T> *im, T alpha, unsigned long long *stats)
+void launch_segsum(Impl<T> *im, T alpha, unsigned long long *stats, hipEvent_t e0 = nullptr,
+                   hipEvent_t e1 = nullptr)
 {
     switch (im->cfg.items_per_lane) {
-    case 4: launch_segsum_s<T, 4, STATS>(im, alpha, stats); break;
-    case 8: launch_segsum_s<T, 8, STATS>(im, alpha, stats); break;
-    case 16: launch_segsum_s<T, 16, STATS>(im, alpha, stats); break;
-    default: launch_segsum_s<T, 32, STATS>(im, alpha, stats); break;
+    case 4: launch_segsum_s<T, 4, STATS>(im, alpha, stats, e0, e1); break;
+    case 8: launch_segsum_s<T, 8, STATS>(im, alpha, stats, e0, e1); break;
+    case 16: launch_segsum_s<T, 16, STATS>(im, alpha, stats, e0, e1); break;
+    default: launch_segsum_s<T, 32, STATS>(im, alpha, stats, e0, e1); break;
     }
 }
 
@@ -488,28 +499,34 @@ inline int tpv_rule(int m, int nnz)
 }
 
 template <typename T, int TPV>
-void launch_csrvec_t(Impl<T> *im, T alpha)
+void launch_csrvec_t(Impl<T> *im, T alpha, hipEvent_t e0, hipEvent_t e1)
 {
     const int threads = 256;
     const long long vpb = threads / TPV;
     long long blocks = ((long long)im->m + vpb - 1) / vpb;
     blocks = std::max<long long>(1, std::min<long long>(blocks, 256LL * 32));
-    hipLaunchKernelGGL((bhs::csrvec_kernel<T, TPV>), dim3((unsigned)blocks), dim3(threads), 0,
-                       im->stream, im->m, im->d_row_ptr, im->d_col, im->d_val, im->d_x, im->d_y,
-                       alpha);
+    if (e0 && e1)
+        hipExtLaunchKernelGGL((bhs::csrvec_kernel<T, TPV>), dim3((unsigned)blocks), dim3(threads),
+                              0u, im->stream, e0, e1, 0u, im->m, (const int *)im->d_row_ptr,
+                              (const int *)im->d_col, (const T *)im->d_val, (const T *)im->d_x,
+                              im->d_y, alpha);
+    else
+        hipLaunchKernelGGL((bhs::csrvec_kernel<T, TPV>), dim3((unsigned)blocks), dim3(threads), 0,
+                           im->stream, im->m, im->d_row_ptr, im->d_col, im->d_val, im->d_x,
+                           im->d_y, alpha);
 }
 
 template <typename T>
-void launch_csrvec(Impl<T> *im, T alpha)
+void launch_csrvec(Impl<T> *im, T alpha, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr)
 {
     const int tpv = im->cfg.tpv ? im->cfg.tpv : tpv_rule(im->m, im->nnz);
     switch (tpv) {
-    case 2: launch_csrvec_t<T, 2>(im, alpha); break;
-    case 4: launch_csrvec_t<T, 4>(im, alpha); break;
-    case 8: launch_csrvec_t<T, 8>(im, alpha); break;
-    case 16: launch_csrvec_t<T, 16>(im, alpha); break;
-    case 32: launch_csrvec_t<T, 32>(im, alpha); break;
-    default: launch_csrvec_t<T, 64>(im, alpha); break;
+    case 2: launch_csrvec_t<T, 2>(im, alpha, e0, e1); break;
+    case 4: launch_csrvec_t<T, 4>(im, alpha, e0, e1); break;
+    case 8: launch_csrvec_t<T, 8>(im, alpha, e0, e1); break;
+    case 16: launch_csrvec_t<T, 16>(im, alpha, e0, e1); break;
+    case 32: launch_csrvec_t<T, 32>(im, alpha, e0, e1); break;
+    default: launch_csrvec_t<T, 64>(im, alpha, e0, e1); break;
     }
 }
 
@@ -530,35 +547,40 @@ size_t segsum_lds_bytes(const bhs_config &c)
     return per_wave * (size_t)c.waves_per_block;
 }
 
-// Enqueues one y = alpha*A*x.  `main_done` (optional) is recorded right after
-// the main kernel.
+// Enqueues one y = alpha*A*x.  If main_start and main_done are given they are
+// attached to the main kernel's dispatch and time exactly that kernel (they
+// are recorded around the fill when there is no kernel to time).
 template <typename T>
-int enqueue_spmv(Impl<T> *im, T alpha, hipEvent_t main_done = nullptr)
+int enqueue_spmv(Impl<T> *im, T alpha, hipEvent_t main_start = nullptr,
+                 hipEvent_t main_done = nullptr)
 {
     if (!im->data) return set_error(BHSPARSE_NOT_INITIALIZED, "init_data first");
     if (int rc = ensure_analysis(im)) return rc;
+    const bool timed = main_start && main_done;
     if (im->m == 0) {
-        if (main_done) BHS_HIP(hipEventRecord(main_done, im->stream));
+        if (timed) {
+            BHS_HIP(hipEventRecord(main_start, im->stream));
+            BHS_HIP(hipEventRecord(main_done, im->stream));
+        }
         return BHSPARSE_SUCCESS;
     }
     if (im->cfg.variant == BHS_VARIANT_CSRVEC) {
-        launch_csrvec(im, alpha);
+        launch_csrvec(im, alpha, main_start, main_done);
         BHS_HIP(hipGetLastError());
-        if (main_done) BHS_HIP(hipEventRecord(main_done, im->stream));
         return BHSPARSE_SUCCESS;
     }
     if (im->NP == 0) {                        // no nonzeros: y = 0
+        if (timed) BHS_HIP(hipEventRecord(main_start, im->stream));
         launch_fill(im, im->d_y, im->m);
         BHS_HIP(hipGetLastError());
-        if (main_done) BHS_HIP(hipEventRecord(main_done, im->stream));
+        if (timed) BHS_HIP(hipEventRecord(main_done, im->stream));
         return BHSPARSE_SUCCESS;
     }
     if (segsum_lds_bytes<T>(im->cfg) > im->max_lds)
         return set_error(BHSPARSE_INVALID_ARGUMENT,
                          "items_per_lane x waves_per_block needs more LDS than a block may use");
-    launch_segsum<T, false>(im, alpha, nullptr);
+    launch_segsum<T, false>(im, alpha, nullptr, main_start, main_done);
     BHS_HIP(hipGetLastError());
-    if (main_done) BHS_HIP(hipEventRecord(main_done, im->stream));
     launch_calibrate(im);
     BHS_HIP(hipGetLastError());
     launch_combine(im);
@@ -757,28 +779,42 @@ int bench(bhs_handle_t h, T alpha, int warmup, int reps, long long flush_bytes, 
         if (int rc = enqueue_spmv(im, alpha)) return rc;
     }
     BHS_HIP(hipStreamSynchronize(im->stream));
-    for (int i = 0; i < reps; ++i) {
+    // whole calls: two events around each, nothing between its kernels
+    for (int i = 0; ms_total && i < reps; ++i) {
         flush();
         BHS_HIP(hipEventRecord(im->ev[0], im->stream));
-        if (int rc = enqueue_spmv(im, alpha, im->ev[1])) return rc;
+        if (int rc = enqueue_spmv(im, alpha)) return rc;
         BHS_HIP(hipEventRecord(im->ev[2], im->stream));
         BHS_HIP(hipEventSynchronize(im->ev[2]));
-        float a = 0, b = 0;
+        float a = 0;
         BHS_HIP(hipEventElapsedTime(&a, im->ev[0], im->ev[2]));
-        BHS_HIP(hipEventElapsedTime(&b, im->ev[0], im->ev[1]));
-        if (ms_total) ms_total[i] = a;
-        if (ms_main) ms_main[i] = b;
+        ms_total[i] = a;
+    }
+    // the main kernel alone: events attached to its dispatch, separate calls
+    for (int i = 0; ms_main && i < reps; ++i) {
+        flush();
+        if (int rc = enqueue_spmv(im, alpha, im->ev[1], im->ev[3])) return rc;
+        BHS_HIP(hipEventSynchronize(im->ev[3]));
+        BHS_HIP(hipStreamSynchronize(im->stream));
+        float b = 0;
+        BHS_HIP(hipEventElapsedTime(&b, im->ev[1], im->ev[3]));
+        ms_main[i] = b;
     }
     return BHSPARSE_SUCCESS;
 }
 
 template <typename T, int S>
-void launch_floor_s(Impl<T> *im)
+void launch_floor_s(Impl<T> *im, hipEvent_t e0, hipEvent_t e1)
 {
     const long long threads = (long long)im->NP * 64;
-    hipLaunchKernelGGL((bhs::floor_kernel<T, S>), dim3((unsigned)((threads + 255) / 256)),
-                       dim3(256), 0, im->stream, active_layout(im).col, active_layout(im).val,
-                       im->d_x, im->NP, im->d_floor);
+    const Layout<T> L = active_layout(im);
+    if (e0 && e1)
+        hipExtLaunchKernelGGL((bhs::floor_kernel<T, S>), dim3((unsigned)((threads + 255) / 256)),
+                              dim3(256), 0u, im->stream, e0, e1, 0u, L.col, L.val,
+                              (const T *)im->d_x, im->NP, im->d_floor);
+    else
+        hipLaunchKernelGGL((bhs::floor_kernel<T, S>), dim3((unsigned)((threads + 255) / 256)),
+                           dim3(256), 0, im->stream, L.col, L.val, im->d_x, im->NP, im->d_floor);
 }
 
 template <typename T>
@@ -809,26 +845,25 @@ int bench_floor(bhs_handle_t h, int reps, long long flush_bytes, double *ms)
         BHS_HIP(hipMalloc(&im->d_flush, (size_t)n16 * 16));
         im->flush_cap = n16 * 16;
     }
-    auto launch = [&]() {
+    auto launch = [&](hipEvent_t e0, hipEvent_t e1) {
         switch (im->cfg.items_per_lane) {
-        case 4: launch_floor_s<T, 4>(im); break;
-        case 8: launch_floor_s<T, 8>(im); break;
-        case 16: launch_floor_s<T, 16>(im); break;
-        default: launch_floor_s<T, 32>(im); break;
+        case 4: launch_floor_s<T, 4>(im, e0, e1); break;
+        case 8: launch_floor_s<T, 8>(im, e0, e1); break;
+        case 16: launch_floor_s<T, 16>(im, e0, e1); break;
+        default: launch_floor_s<T, 32>(im, e0, e1); break;
         }
     };
-    launch();
+    launch(nullptr, nullptr);
     BHS_HIP(hipGetLastError());
     for (int i = 0; i < reps; ++i) {
         if (n16 > 0)
             hipLaunchKernelGGL(bhs::flush_kernel, dim3(4096), dim3(256), 0, im->stream,
                                reinterpret_cast<bhs::u32x4 *>(im->d_flush), n16, 0u);
-        BHS_HIP(hipEventRecord(im->ev[0], im->stream));
-        launch();
-        BHS_HIP(hipEventRecord(im->ev[1], im->stream));
-        BHS_HIP(hipEventSynchronize(im->ev[1]));
+        launch(im->ev[1], im->ev[3]);
+        BHS_HIP(hipGetLastError());
+        BHS_HIP(hipEventSynchronize(im->ev[3]));
         float t = 0;
-        BHS_HIP(hipEventElapsedTime(&t, im->ev[0], im->ev[1]));
+        BHS_HIP(hipEventElapsedTime(&t, im->ev[1], im->ev[3]));
         ms[i] = t;
     }
     return BHSPARSE_SUCCESS;

@@ -108,6 +108,29 @@ __device__ __forceinline__ RowChunk load_chunk(const int *__restrict__ row_ptr, 
     return RowChunk{row_ptr[i0], row_ptr[i0 + 1], row_ptr[i1], row_ptr[i1 + 1]};
 }
 
+// Given row_ptr[lo] == v (rows before lo, back to the run's start, are empty),
+// the end (exclusive, at most stop+1) of the run of empty rows: the first row
+// at or after lo with entries.  Wave-uniform doubling steps over row_ptr,
+// closed by bisection, so a tile spanning a long run of empty rows costs
+// O(log run) loads instead of one per row.
+__device__ __forceinline__ int empty_run_end(const int *__restrict__ row_ptr, int lo, int v,
+                                             int stop)
+{
+    int step = 64;
+    while (lo + step <= stop + 1 && row_ptr[lo + step] == v) {
+        lo += step;
+        step *= 2;
+    }
+    int hi = min(lo + step, stop + 1);
+    if (row_ptr[hi] == v) return stop + 1;   // empty up to the tile's last row
+    while (hi - lo > 1) {                    // row_ptr[lo] == v < row_ptr[hi]
+        const int mid = lo + (hi - lo) / 2;
+        if (row_ptr[mid] == v) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // ---------------------------------------------------------------------------
 // Speculative segmented sum (kernels_cuda.h:7-269, wave64 form).
 //
@@ -280,15 +303,29 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                 for (int rr = 0; rr < k; ++rr)
                     y[(long long)(r0 + kWave + lane) * ks + rr] = T(0);
             }
-            for (int r = r0 + 2 * kWave + lane; r <= stop; r += kWave) {   // rare: > 128 rows
-                const int rp = row_ptr[r], rpn = row_ptr[r + 1];
-                if (rpn > rp) {
-                    const int o = rp - B;
-                    if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
-                } else {
+            for (int rb = r0 + 2 * kWave; rb <= stop;) {   // rare: > 128 rows
+                const int rq = min(rb + lane, stop);
+                const int rp = row_ptr[rq], rpn = row_ptr[rq + 1];
+                const int vb = row_ptr[min(rb + kWave, stop + 1)];
+                if (rb + kWave <= stop && read_lane(rp, 0) == vb) {
+                    // 64 empty rows with more rows after them: skip the whole run
+                    const int re = empty_run_end(row_ptr, rb + kWave, vb, stop);
                     lane_dirty = true;
-                    for (int rr = 0; rr < k; ++rr) y[(long long)r * ks + rr] = T(0);
+                    for (int r = rb + lane; r < re; r += kWave)
+                        for (int rr = 0; rr < k; ++rr) y[(long long)r * ks + rr] = T(0);
+                    rb = re;
+                    continue;
                 }
+                if (rb + lane <= stop) {
+                    if (rpn > rp) {
+                        const int o = rp - B;
+                        if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
+                    } else {
+                        lane_dirty = true;
+                        for (int rr = 0; rr < k; ++rr) y[(long long)(rb + lane) * ks + rr] = T(0);
+                    }
+                }
+                rb += kWave;
             }
             const bool dirty = ballot(lane_dirty) != 0ull;
             wave_sync();
@@ -372,15 +409,22 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                     if (ne1 && rank1 >= kb && rank1 < ke)
                         y[(long long)(r0 + kWave + lane) * ks + rr] = alpha * s_buf[rank1];
                     int running = __popcll(m0) + __popcll(m1);
-                    for (int rb = r0 + 2 * kWave; rb <= stop; rb += kWave) {   // rare
+                    for (int rb = r0 + 2 * kWave; rb <= stop;) {   // rare: > 128 rows
+                        const int rq = min(rb + lane, stop);
+                        const int rp = row_ptr[rq], rpn = row_ptr[rq + 1];
+                        const int vb = row_ptr[min(rb + kWave, stop + 1)];
+                        if (rb + kWave <= stop && read_lane(rp, 0) == vb) {
+                            rb = empty_run_end(row_ptr, rb + kWave, vb, stop);   // no segments
+                            continue;
+                        }
                         const int r = rb + lane;
-                        bool ne = false;
-                        if (r <= stop) ne = row_ptr[r + 1] > row_ptr[r];
+                        const bool ne = r <= stop && rpn > rp;
                         const uint64_t msk = ballot(ne);
                         const int rank = running + popc_below(msk);
                         if (ne && rank >= kb && rank < ke)
                             y[(long long)r * ks + rr] = alpha * s_buf[rank];
                         running += __popcll(msk);
+                        rb += kWave;
                     }
                 }
                 if (syncing && lane == 0) mv.sync_val[(long long)w * KMAX + rr] = alpha * s_buf[0];

@@ -17,7 +17,10 @@ constexpr int kMaxClasses = 64;
 
 // raw = {m, nnz, mean row length, row-length variance, max row length,
 // empty-row fraction, mean column span of nonempty rows, 1 if fp64 else 0}
-// -> scale-free inputs of comparable range.
+// -> scale-free inputs of comparable range.  The span enters relative to m
+// (the x window a row touches, as a fraction of x for a square matrix): it
+// separates clustered columns (stencils, bands) from scattered ones whatever
+// the matrix size.
 __host__ __device__ inline void tuner_transform(const double *raw, float *f)
 {
     f[0] = (float)(log10(raw[0] > 1 ? raw[0] : 1.0) / 7.0);
@@ -26,7 +29,7 @@ __host__ __device__ inline void tuner_transform(const double *raw, float *f)
     f[3] = (float)(log2(sqrt(raw[3] > 0 ? raw[3] : 0.0) + 1.0) / 10.0);
     f[4] = (float)(log2(raw[4] + 1.0) / 17.0);
     f[5] = (float)raw[5];
-    f[6] = (float)(log2(raw[6] + 1.0) / 23.0);
+    f[6] = (float)(log2((raw[6] + 1.0) / ((raw[0] > 1 ? raw[0] : 1.0) + 1.0)) / 23.0);
     f[7] = (float)raw[7];
 }
 

@@ -154,8 +154,9 @@ def _rng(seed):
 def family_matrix(seed: int, m_lo: float, m_hi: float, family: int | None = None) -> CSR:
     """One matrix of the four mixed families: m log-uniform in [m_lo, m_hi],
     family = seed mod 4 unless given:
-    0 uniform rows in [0, L], L in [4, 64]; 1 power law alpha in [1.5, 3];
-    2 banded, bandwidth 3..201; 3 3x3 block rows."""
+    0 uniform rows in [0, L], L in [4, 64]; 1 power law alpha in [1.5, 3] with
+    a fraction in [0, 0.2] of the rows empty; 2 banded, bandwidth 3..201;
+    3 3x3 block rows."""
     g = _rng(seed)
     m = int(10 ** g.uniform(np.log10(m_lo), np.log10(m_hi)))
     fam = seed % 4 if family is None else family
@@ -164,7 +165,8 @@ def family_matrix(seed: int, m_lo: float, m_hi: float, family: int | None = None
         A = uniform_rows(seed, m, m, 0, L)
     elif fam == 1:
         a = float(g.uniform(1.5, 3.0))
-        A = power_law(seed, m, m, a, min(100000, m), 0.0, 0)
+        ef = float(g.uniform(0.0, 0.2))
+        A = power_law(seed, m, m, a, min(100000, m), ef, 0)
     elif fam == 2:
         bw = int(g.integers(1, 101)) * 2 + 1
         A = banded(seed, m, m, bw)

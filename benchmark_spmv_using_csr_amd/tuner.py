@@ -292,6 +292,8 @@ def main(argv=None):
     s = sub.add_parser("sweep")
     s.add_argument("--seeds", default="0:2000")
     s.add_argument("--reps", type=int, default=7)
+    s.add_argument("--family", type=int, default=-1,
+                   help="sweep only the seeds of this family (seed mod 4)")
     s.add_argument("--out", required=True)
     t = sub.add_parser("train")
     t.add_argument("--data", nargs="+", required=True)
@@ -304,7 +306,8 @@ def main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "sweep":
         lo, hi = (int(v) for v in a.seeds.split(":"))
-        d = sweep(range(lo, hi), reps=a.reps, log=lambda m: print(m, flush=True))
+        seeds = [s_ for s_ in range(lo, hi) if a.family < 0 or s_ % 4 == a.family]
+        d = sweep(seeds, reps=a.reps, log=lambda m: print(m, flush=True))
         np.savez(a.out, **d)
         print(f"wrote {a.out}", flush=True)
     elif a.cmd == "train":

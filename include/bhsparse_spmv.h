@@ -245,7 +245,8 @@ int bhs_s_batch_destroy(bhs_batch_t b);
  * inputs are standardised as (x - mean) / std inside the network. */
 long long bhs_mlp_param_count(int n_in, int n_hidden, int n_out);
 /* raw = {m, nnz, mean len, var len, max len, empty fraction, mean column span,
- * 1 if fp64}: the transform the tuner applies before standardising. */
+ * 1 if fp64}: the transform the tuner applies before standardising (logs of
+ * the sizes and lengths, the span as a fraction of m). */
 int bhs_tuner_transform(const double *raw8, float *out8);
 int bhs_mlp_create(bhs_mlp_t *net, int device, int n_in, int n_hidden, int n_out);
 int bhs_mlp_destroy(bhs_mlp_t net);

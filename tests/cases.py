@@ -98,6 +98,13 @@ def adversarial(P, T, seed=0, integer=False):
     add("empty rows at tile boundary", [P // 2, 0, 0, P // 2, 0, P, 0, 1])
     add("dirty last tile", [5] * (P // 5) + [1, 0, 2, 0, 0])
     add("huge empty run", [3] + [0] * 5000 + [4] + [0] * 300 + [2])
+    # runs of empty rows around the 64- and 128-row chunk sizes, a run far
+    # longer than a tile inside it, and one reaching the last row
+    add("empty runs of 63/64/65/127/128/129", [2] + [0] * 63 + [3] + [0] * 64 + [1] + [0] * 65 +
+        [2] + [0] * 127 + [1] + [0] * 128 + [3] + [0] * 129 + [2])
+    add("100k empty rows inside a tile", [5] * 40 + [0] * 100000 + [3] * 30 + [0] * 700 + [1] * 9,
+        n=512)
+    add("empty run to the last row", [4] * 20 + [0] * 20000, n=256)
     add("all rows length 1", [1] * (3 * P + 11))
     add("alternating 0/1", [1, 0] * (2 * P))
     add("rectangular n > m", rng.integers(0, 40, 200), n=1000)

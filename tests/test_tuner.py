@@ -70,5 +70,5 @@ def test_transform_matches_formula():
     raw = np.array([1e6, 1e7, 10.0, 400.0, 1e5, 0.1, 5e5, 1.0])
     f = tuner.transform(raw)[0]
     want = [np.log10(1e6) / 7, np.log10(1e7 + 1) / 9, np.log2(11) / 8, np.log2(21) / 10,
-            np.log2(1e5 + 1) / 17, 0.1, np.log2(5e5 + 1) / 23, 1.0]
+            np.log2(1e5 + 1) / 17, 0.1, np.log2((5e5 + 1) / (1e6 + 1)) / 23, 1.0]
     np.testing.assert_allclose(f, want, rtol=1e-6)
