@@ -92,6 +92,7 @@ _PER_TYPE = {
     "bench": [_H, "V", C.c_int, C.c_int, C.c_longlong, C.c_void_p, C.c_void_p],
     "row_features": [_H, C.c_void_p],
     "bench_floor": [_H, C.c_int, C.c_longlong, C.c_void_p],
+    "bench_event_overhead": [_H, C.c_int, C.c_longlong, C.c_void_p],
     "autotune": [_H, C.POINTER(Config), C.c_int, C.c_int, C.c_longlong, C.c_void_p,
                  C.POINTER(C.c_int)],
     "apply_tuner": [_H, C.c_void_p, C.POINTER(Config), C.c_int, C.POINTER(C.c_int)],

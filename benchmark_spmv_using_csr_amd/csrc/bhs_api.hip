@@ -817,6 +817,41 @@ void launch_floor_s(Impl<T> *im, hipEvent_t e0, hipEvent_t e1)
                            dim3(256), 0, im->stream, L.col, L.val, im->d_x, im->NP, im->d_floor);
 }
 
+// The fixed cost of the timing events themselves: an interval between the two
+// event slots that bench() attaches to the main kernel, with nothing in it,
+// after the same cache flush.  Subtracting its median from ms_main gives the
+// kernel's own duration (what a kernel trace reports).
+template <typename T>
+int bench_event_overhead(bhs_handle_t h, int reps, long long flush_bytes, double *ms)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (!im->platform) return set_error(BHSPARSE_NOT_INITIALIZED, "init_platform first");
+    if (reps < 1 || !ms || flush_bytes < 0)
+        return set_error(BHSPARSE_INVALID_ARGUMENT, "bad reps/output");
+    BHS_HIP(hipSetDevice(im->device));
+    const long long n16 = flush_bytes / 16;
+    if (n16 > 0 && flush_bytes > im->flush_cap) {
+        if (im->d_flush) (void)hipFree(im->d_flush);
+        im->d_flush = nullptr;
+        im->flush_cap = 0;
+        BHS_HIP(hipMalloc(&im->d_flush, (size_t)n16 * 16));
+        im->flush_cap = n16 * 16;
+    }
+    for (int i = 0; i < reps; ++i) {
+        if (n16 > 0)
+            hipLaunchKernelGGL(bhs::flush_kernel, dim3(4096), dim3(256), 0, im->stream,
+                               reinterpret_cast<bhs::u32x4 *>(im->d_flush), n16, 0u);
+        BHS_HIP(hipEventRecord(im->ev[1], im->stream));
+        BHS_HIP(hipEventRecord(im->ev[3], im->stream));
+        BHS_HIP(hipEventSynchronize(im->ev[3]));
+        float t = 0;
+        BHS_HIP(hipEventElapsedTime(&t, im->ev[1], im->ev[3]));
+        ms[i] = t;
+    }
+    return BHSPARSE_SUCCESS;
+}
+
 template <typename T>
 int bench_floor(bhs_handle_t h, int reps, long long flush_bytes, double *ms)
 {
@@ -1101,6 +1136,10 @@ int bhs_stream_copy(int device, long long bytes, int reps, double *ms)
     int bhs_##P##_bench_floor(bhs_handle_t h, int r, long long fb, double *ms)                \
     {                                                                                          \
         return bench_floor<T>(h, r, fb, ms);                                                   \
+    }                                                                                          \
+    int bhs_##P##_bench_event_overhead(bhs_handle_t h, int r, long long fb, double *ms)       \
+    {                                                                                          \
+        return bench_event_overhead<T>(h, r, fb, ms);                                          \
     }                                                                                          \
     int bhs_##P##_autotune(bhs_handle_t h, const bhs_config *c, int n, int r, long long fb,    \
                            double *us, int *best)                                              \

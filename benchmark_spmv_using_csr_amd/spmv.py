@@ -141,6 +141,14 @@ class bhsparse_spmv:  # noqa: N801  (reference class name)
                                       ms.ctypes.data_as(C.c_void_p)), "bench_floor")
         return ms[:reps]
 
+    def bench_event_overhead(self, reps=20, flush_bytes=0) -> np.ndarray:
+        """Per-rep ms of an empty interval between the main-kernel timing events."""
+        ms = np.zeros(max(reps, 1), np.float64)
+        check(self._fn("bench_event_overhead")(self._h, int(reps), int(flush_bytes),
+                                               ms.ctypes.data_as(C.c_void_p)),
+              "bench_event_overhead")
+        return ms[:reps]
+
     def autotune(self, classes, reps=7, flush_bytes=1 << 30):
         """Exhaustive autotuner: (us per class, -1 where a class cannot run;
         index of the fastest, which is installed)."""

@@ -142,10 +142,12 @@ int bhs_s_get_config(bhs_handle_t h, bhs_config *cfg);
 int bhs_s_get_partition(bhs_handle_t h, int *tile_row, int cap, int *np);
 /* Debug: runs one counting launch of the segmented sum and reports its counters. */
 int bhs_s_get_stats(bhs_handle_t h, bhs_spmv_stats *out);
-/* Timing: `warmup` untimed calls, then `reps` calls each bracketed by HIP
- * events on the handle's stream.  If flush_bytes > 0, a buffer of that size is
- * written before every call (outside the events) so caches start cold.
- * ms_total[i] = whole call, ms_main[i] = main kernel only (both may be NULL). */
+/* Timing: `warmup` untimed calls, then `reps` calls each bracketed by two HIP
+ * events on the handle's stream (nothing between the call's kernels).  If
+ * flush_bytes > 0, a buffer of that size is written before every call (outside
+ * the events) so caches start cold.  ms_total[i] = whole call; ms_main[i] =
+ * main kernel only, from a second pass of calls with events attached to that
+ * kernel's dispatch (both may be NULL). */
 int bhs_s_bench(bhs_handle_t h, float alpha, int warmup, int reps, long long flush_bytes,
                 double *ms_total, double *ms_main);
 /* GPU reduction over row_ptr/col: 8 row-length features (m, nnz, mean, var,
@@ -158,6 +160,11 @@ int bhs_s_row_features(bhs_handle_t h, double *feat8);
  * to bhs_s_bench's says how close the SpMV is to what the matrix's own memory
  * traffic allows on the device. */
 int bhs_s_bench_floor(bhs_handle_t h, int reps, long long flush_bytes, double *ms);
+/* Timing calibration (no reference counterpart): ms[i] = an empty interval
+ * between the two events bhs_s_bench uses for ms_main, after the same flush.
+ * Those events cost a few microseconds of stream time; ms_main minus the
+ * median of this is the kernel's own duration, as a kernel trace reports it. */
+int bhs_s_bench_event_overhead(bhs_handle_t h, int reps, long long flush_bytes, double *ms);
 /* Exhaustive autotuner (no reference counterpart; the reference fixes its tile
  * shape at compile time, spmv_cuda/common.h:39-48): times every candidate
  * (median of `reps` calls, caches flushed when flush_bytes > 0), writes us[i]
@@ -191,6 +198,7 @@ int bhs_d_bench(bhs_handle_t h, double alpha, int warmup, int reps, long long fl
                 double *ms_total, double *ms_main);
 int bhs_d_row_features(bhs_handle_t h, double *feat8);
 int bhs_d_bench_floor(bhs_handle_t h, int reps, long long flush_bytes, double *ms);
+int bhs_d_bench_event_overhead(bhs_handle_t h, int reps, long long flush_bytes, double *ms);
 int bhs_d_autotune(bhs_handle_t h, const bhs_config *cands, int n_cands, int reps,
                    long long flush_bytes, double *us, int *best);
 int bhs_d_apply_tuner(bhs_handle_t h, bhs_mlp_t net, const bhs_config *classes, int n_classes,
