@@ -108,6 +108,12 @@ __device__ __forceinline__ RowChunk load_chunk(const int *__restrict__ row_ptr, 
     return RowChunk{row_ptr[i0], row_ptr[i0 + 1], row_ptr[i1], row_ptr[i1 + 1]};
 }
 
+// Compile-time switch between the tile code paths of segsum_wave.
+template <bool B>
+struct BigTag {
+    static constexpr bool value = B;
+};
+
 // Given row_ptr[lo] == v (rows before lo, back to the run's start, are empty),
 // the end (exclusive, at most stop+1) of the run of empty rows: the first row
 // at or after lo with entries.  Wave-uniform doubling steps over row_ptr,
@@ -215,6 +221,12 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
     const int r_first = tile_row[p_begin];
     bool syncing = false;                       // first row began in an earlier wave
     bool did_sync = false;
+    // The wave's one boundary-row partial (KMAX == 1) stays in registers and is
+    // stored after the tile loop: a store inside the loop would make the
+    // compiler wait for it (and, the counter being in order, for every load
+    // issued since) before its data register is reused.
+    int sync_key = -1;
+    T sync_v1 = T(0);
     bool open = false;                          // a row continues past the last tile
     T carry1 = T(0);                            // the open row's partial (KMAX == 1)
     if (KMAX > 1) {
@@ -280,162 +292,181 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
             open = true;
             if (STATS) ++n_fast;
         } else {
-            // ---- bit flags (once for all right-hand sides) ------------------
-            const int stop = (r1 == mv.m) ? mv.m - 1 : r1;
-            const bool have0 = r0 + lane <= stop;
-            const bool have1 = r0 + kWave + lane <= stop;
-            const bool ne0 = have0 && cc.b0 > cc.a0;
-            const bool ne1 = have1 && cc.b1 > cc.a1;
-            bool lane_dirty = (have0 && !ne0) || (have1 && !ne1);
-            wave_sync();                         // previous tile's LDS reads are done
-            s_head[lane] = 0u;
-            wave_sync();
-            if (ne0) {
-                const int o = cc.a0 - B;
-                if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
-            } else if (have0) {
-                for (int rr = 0; rr < k; ++rr) y[(long long)(r0 + lane) * ks + rr] = T(0);
-            }
-            if (ne1) {
-                const int o = cc.a1 - B;
-                if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
-            } else if (have1) {
-                for (int rr = 0; rr < k; ++rr)
-                    y[(long long)(r0 + kWave + lane) * ks + rr] = T(0);
-            }
-            for (int rb = r0 + 2 * kWave; rb <= stop;) {   // rare: > 128 rows
-                const int rq = min(rb + lane, stop);
-                const int rp = row_ptr[rq], rpn = row_ptr[rq + 1];
-                const int vb = row_ptr[min(rb + kWave, stop + 1)];
-                if (rb + kWave <= stop && read_lane(rp, 0) == vb) {
-                    // 64 empty rows with more rows after them: skip the whole run
-                    const int re = empty_run_end(row_ptr, rb + kWave, vb, stop);
-                    lane_dirty = true;
-                    for (int r = rb + lane; r < re; r += kWave)
-                        for (int rr = 0; rr < k; ++rr) y[(long long)r * ks + rr] = T(0);
-                    rb = re;
-                    continue;
+            // A tile of at most 128 rows takes every row boundary from its
+            // chunk; only longer tiles load more row_ptr values, in loops.
+            // The two cases are separate instantiations, so the loops (and
+            // the waits they need) are not on the common path at all.
+            auto general_tile = [&](auto big) __attribute__((always_inline)) {
+                constexpr bool BIG = decltype(big)::value;
+                // ---- bit flags (once for all right-hand sides) ------------------
+                const int stop = (r1 == mv.m) ? mv.m - 1 : r1;
+                const bool have0 = r0 + lane <= stop;
+                const bool have1 = r0 + kWave + lane <= stop;
+                const bool ne0 = have0 && cc.b0 > cc.a0;
+                const bool ne1 = have1 && cc.b1 > cc.a1;
+                bool lane_dirty = (have0 && !ne0) || (have1 && !ne1);
+                wave_sync();                         // previous tile's LDS reads are done
+                s_head[lane] = 0u;
+                wave_sync();
+                // (empty rows of the chunk get their zero in the output step,
+                // after this tile's loads have been consumed)
+                if (ne0) {
+                    const int o = cc.a0 - B;
+                    if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
                 }
-                if (rb + lane <= stop) {
-                    if (rpn > rp) {
-                        const int o = rp - B;
-                        if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
-                    } else {
+                if (ne1) {
+                    const int o = cc.a1 - B;
+                    if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
+                }
+                if constexpr (BIG)
+                for (int rb = r0 + 2 * kWave; rb <= stop;) {   // > 128 rows
+                    const int rq = min(rb + lane, stop);
+                    const int rp = row_ptr[rq], rpn = row_ptr[rq + 1];
+                    const int vb = row_ptr[min(rb + kWave, stop + 1)];
+                    if (rb + kWave <= stop && read_lane(rp, 0) == vb) {
+                        // 64 empty rows with more rows after them: skip the whole run
+                        const int re = empty_run_end(row_ptr, rb + kWave, vb, stop);
                         lane_dirty = true;
-                        for (int rr = 0; rr < k; ++rr) y[(long long)(rb + lane) * ks + rr] = T(0);
+                        for (int r = rb + lane; r < re; r += kWave)
+                            for (int rr = 0; rr < k; ++rr) y[(long long)r * ks + rr] = T(0);
+                        rb = re;
+                        continue;
                     }
-                }
-                rb += kWave;
-            }
-            const bool dirty = ballot(lane_dirty) != 0ull;
-            wave_sync();
-            unsigned hb = s_head[lane];
-            if (lane == 0) hb |= 1u;
-
-            // ---- segment numbers -----------------------------------------
-            const int nh = __popc(hb);
-            const int incl = wave_incl_scan(nh, lane);
-            const int seg_base = incl - nh;
-            const int K = read_lane(incl, kWave - 1);
-            // row_ptr[r1] sits in the chunk when r1 < r0+128
-            int rp_r1;
-            if (r1 - r0 < kWave) rp_r1 = read_lane(cc.a0, r1 - r0);
-            else if (r1 - r0 < 2 * kWave) rp_r1 = read_lane(cc.a1, r1 - r0 - kWave);
-            else rp_r1 = row_ptr[min(r1, mv.m)];
-            const bool partial_tail = (r1 < mv.m) && (rp_r1 < B + P);
-            const int kb = syncing ? 1 : 0;
-            const int ke = partial_tail ? K - 1 : K;
-            const uint64_t m0 = ballot(ne0);
-            const uint64_t m1 = ballot(ne1);
-            const int rank0 = popc_below(m0);
-            const int rank1 = __popcll(m0) + popc_below(m1);
-            if (STATS && dirty) ++n_dirty;
-
-            for (int rr = 0; rr < k; ++rr) {
-                if (rr > 0) {
-#pragma unroll
-                    for (int kk = 0; kk < S; ++kk) xv[kk] = x[(long long)cur.c[kk] * ks + rr];
-                }
-                T prod[S];
-#pragma unroll
-                for (int kk = 0; kk < S; ++kk) {
-                    const T pk = cur.v[kk] * xv[kk];
-                    prod[kk] = tail ? keep_or_zero(pk, tile_item<T, S>(B, lane, kk) < mv.nnz) : pk;
-                }
-                // products -> LDS in tile order, back lane-contiguous
-                wave_sync();
-#pragma unroll
-                for (int j = 0; j < S / G; ++j) {
-                    T *dst = s_buf + j * (kWave * G) + lane * G;
-#pragma unroll
-                    for (int g = 0; g < G; ++g) dst[g] = prod[j * G + g];
-                }
-                wave_sync();
-                T pc[S];
-#pragma unroll
-                for (int kk = 0; kk < S; ++kk) pc[kk] = s_buf[lane * S + kk];
-                wave_sync();
-                if (lane == 0 && cont) pc[0] = pc[0] + carry_get(rr);
-
-                // ---- lane-sequential segmented reduction -----------------
-                T acc = T(0), lead = T(0);
-                bool seen = false;
-                int sidx = seg_base;
-#pragma unroll
-                for (int kk = 0; kk < S; ++kk) {
-                    if ((hb >> kk) & 1u) {
-                        if (seen) { s_buf[sidx] = acc; ++sidx; }
-                        else { lead = acc; seen = true; }
-                        acc = T(0);
-                    }
-                    acc += pc[kk];
-                }
-
-                // ---- segments spanning lanes -----------------------------
-                const uint64_t present = ballot(seen);  // bit 0 always set
-                const T I = wave_segscan(acc, present, lane);
-                const T Iprev = __shfl_up(I, 1, kWave);
-                if (seen && lane > 0) s_buf[seg_base - 1] = Iprev + lead;
-                if (lane == kWave - 1) s_buf[K - 1] = I;
-                wave_sync();
-
-                // ---- output ------------------------------------------------
-                if (!dirty) {
-                    for (int kk = kb + lane; kk < ke; kk += kWave)
-                        y[(long long)(r0 + kk) * ks + rr] = alpha * s_buf[kk];
-                } else {
-                    if (ne0 && rank0 >= kb && rank0 < ke)
-                        y[(long long)(r0 + lane) * ks + rr] = alpha * s_buf[rank0];
-                    if (ne1 && rank1 >= kb && rank1 < ke)
-                        y[(long long)(r0 + kWave + lane) * ks + rr] = alpha * s_buf[rank1];
-                    int running = __popcll(m0) + __popcll(m1);
-                    for (int rb = r0 + 2 * kWave; rb <= stop;) {   // rare: > 128 rows
-                        const int rq = min(rb + lane, stop);
-                        const int rp = row_ptr[rq], rpn = row_ptr[rq + 1];
-                        const int vb = row_ptr[min(rb + kWave, stop + 1)];
-                        if (rb + kWave <= stop && read_lane(rp, 0) == vb) {
-                            rb = empty_run_end(row_ptr, rb + kWave, vb, stop);   // no segments
-                            continue;
+                    if (rb + lane <= stop) {
+                        if (rpn > rp) {
+                            const int o = rp - B;
+                            if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
+                        } else {
+                            lane_dirty = true;
+                            for (int rr = 0; rr < k; ++rr) y[(long long)(rb + lane) * ks + rr] = T(0);
                         }
-                        const int r = rb + lane;
-                        const bool ne = r <= stop && rpn > rp;
-                        const uint64_t msk = ballot(ne);
-                        const int rank = running + popc_below(msk);
-                        if (ne && rank >= kb && rank < ke)
-                            y[(long long)r * ks + rr] = alpha * s_buf[rank];
-                        running += __popcll(msk);
-                        rb += kWave;
                     }
+                    rb += kWave;
                 }
-                if (syncing && lane == 0) mv.sync_val[(long long)w * KMAX + rr] = alpha * s_buf[0];
-                carry_set(rr, s_buf[K - 1]);
-            }
-            if (syncing) {
-                if (lane == 0) mv.sync_row[w] = mv.key_base + r0 * ks;
-                syncing = false;
-                did_sync = true;
-            }
-            open = partial_tail;
+                const bool dirty = ballot(lane_dirty) != 0ull;
+                wave_sync();
+                unsigned hb = s_head[lane];
+                if (lane == 0) hb |= 1u;
+
+                // ---- segment numbers -----------------------------------------
+                const int nh = __popc(hb);
+                const int incl = wave_incl_scan(nh, lane);
+                const int seg_base = incl - nh;
+                const int K = read_lane(incl, kWave - 1);
+                // row_ptr[r1] sits in the chunk when r1 < r0+128
+                int rp_r1;
+                if (r1 - r0 < kWave) rp_r1 = read_lane(cc.a0, r1 - r0);
+                else if (!BIG || r1 - r0 < 2 * kWave) rp_r1 = read_lane(cc.a1, r1 - r0 - kWave);
+                else rp_r1 = row_ptr[min(r1, mv.m)];
+                const bool partial_tail = (r1 < mv.m) && (rp_r1 < B + P);
+                const int kb = syncing ? 1 : 0;
+                const int ke = partial_tail ? K - 1 : K;
+                const uint64_t m0 = ballot(ne0);
+                const uint64_t m1 = ballot(ne1);
+                const int rank0 = popc_below(m0);
+                const int rank1 = __popcll(m0) + popc_below(m1);
+                if (STATS && dirty) ++n_dirty;
+
+                for (int rr = 0; rr < k; ++rr) {
+                    if (rr > 0) {
+    #pragma unroll
+                        for (int kk = 0; kk < S; ++kk) xv[kk] = x[(long long)cur.c[kk] * ks + rr];
+                    }
+                    T prod[S];
+    #pragma unroll
+                    for (int kk = 0; kk < S; ++kk) {
+                        const T pk = cur.v[kk] * xv[kk];
+                        prod[kk] = tail ? keep_or_zero(pk, tile_item<T, S>(B, lane, kk) < mv.nnz) : pk;
+                    }
+                    // products -> LDS in tile order, back lane-contiguous
+                    wave_sync();
+    #pragma unroll
+                    for (int j = 0; j < S / G; ++j) {
+                        T *dst = s_buf + j * (kWave * G) + lane * G;
+    #pragma unroll
+                        for (int g = 0; g < G; ++g) dst[g] = prod[j * G + g];
+                    }
+                    wave_sync();
+                    T pc[S];
+    #pragma unroll
+                    for (int kk = 0; kk < S; ++kk) pc[kk] = s_buf[lane * S + kk];
+                    wave_sync();
+                    if (lane == 0 && cont) pc[0] = pc[0] + carry_get(rr);
+
+                    // ---- lane-sequential segmented reduction -----------------
+                    T acc = T(0), lead = T(0);
+                    bool seen = false;
+                    int sidx = seg_base;
+    #pragma unroll
+                    for (int kk = 0; kk < S; ++kk) {
+                        if ((hb >> kk) & 1u) {
+                            if (seen) { s_buf[sidx] = acc; ++sidx; }
+                            else { lead = acc; seen = true; }
+                            acc = T(0);
+                        }
+                        acc += pc[kk];
+                    }
+
+                    // ---- segments spanning lanes -----------------------------
+                    const uint64_t present = ballot(seen);  // bit 0 always set
+                    const T I = wave_segscan(acc, present, lane);
+                    const T Iprev = __shfl_up(I, 1, kWave);
+                    if (seen && lane > 0) s_buf[seg_base - 1] = Iprev + lead;
+                    if (lane == kWave - 1) s_buf[K - 1] = I;
+                    wave_sync();
+
+                    // ---- output ------------------------------------------------
+                    if (!dirty) {
+                        for (int kk = kb + lane; kk < ke; kk += kWave)
+                            y[(long long)(r0 + kk) * ks + rr] = alpha * s_buf[kk];
+                    } else {
+                        if (ne0 && rank0 >= kb && rank0 < ke)
+                            y[(long long)(r0 + lane) * ks + rr] = alpha * s_buf[rank0];
+                        else if (have0 && !ne0)
+                            y[(long long)(r0 + lane) * ks + rr] = T(0);
+                        if (ne1 && rank1 >= kb && rank1 < ke)
+                            y[(long long)(r0 + kWave + lane) * ks + rr] = alpha * s_buf[rank1];
+                        else if (have1 && !ne1)
+                            y[(long long)(r0 + kWave + lane) * ks + rr] = T(0);
+                        int running = __popcll(m0) + __popcll(m1);
+                        if constexpr (BIG)
+                        for (int rb = r0 + 2 * kWave; rb <= stop;) {   // > 128 rows
+                            const int rq = min(rb + lane, stop);
+                            const int rp = row_ptr[rq], rpn = row_ptr[rq + 1];
+                            const int vb = row_ptr[min(rb + kWave, stop + 1)];
+                            if (rb + kWave <= stop && read_lane(rp, 0) == vb) {
+                                rb = empty_run_end(row_ptr, rb + kWave, vb, stop);   // no segments
+                                continue;
+                            }
+                            const int r = rb + lane;
+                            const bool ne = r <= stop && rpn > rp;
+                            const uint64_t msk = ballot(ne);
+                            const int rank = running + popc_below(msk);
+                            if (ne && rank >= kb && rank < ke)
+                                y[(long long)r * ks + rr] = alpha * s_buf[rank];
+                            running += __popcll(msk);
+                            rb += kWave;
+                        }
+                    }
+                    if (syncing) {
+                        if constexpr (KMAX == 1) sync_v1 = alpha * s_buf[0];
+                        else if (lane == 0) mv.sync_val[(long long)w * KMAX + rr] = alpha * s_buf[0];
+                    }
+                    carry_set(rr, s_buf[K - 1]);
+                }
+                if (syncing) {
+                    sync_key = mv.key_base + r0 * ks;
+                    syncing = false;
+                    did_sync = true;
+                }
+                open = partial_tail;
+                // the long-tile path ends with nothing in flight, so registers
+                // its row_ptr loads used carry no pending write into the next
+                // tile (which would otherwise wait for every load in flight)
+                if constexpr (BIG) __builtin_amdgcn_s_waitcnt(0);
+            };
+            if (r1 - r0 < 2 * kWave) general_tile(BigTag<false>{});
+            else general_tile(BigTag<true>{});
         }
         cur = nxt;
     }
@@ -443,17 +474,20 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
     if (open) {
         const int r_open = tile_row[p_end];
         if (syncing) {
-            if (lane == 0) {
-                mv.sync_row[w] = mv.key_base + r_open * ks;
+            sync_key = mv.key_base + r_open * ks;
+            if constexpr (KMAX == 1) sync_v1 = alpha * carry_get(0);
+            else if (lane == 0)
                 for (int rr = 0; rr < k; ++rr)
                     mv.sync_val[(long long)w * KMAX + rr] = alpha * carry_get(rr);
-            }
             did_sync = true;
         } else if (lane == 0) {
             for (int rr = 0; rr < k; ++rr) y[(long long)r_open * ks + rr] = alpha * carry_get(rr);
         }
     }
-    if (!did_sync && lane == 0) mv.sync_row[w] = -1;
+    if (lane == 0) {
+        mv.sync_row[w] = did_sync ? sync_key : -1;
+        if (KMAX == 1 && did_sync) mv.sync_val[w] = sync_v1;
+    }
     if (STATS && lane == 0) {
         atomicAdd(&stats[0], n_fast);
         atomicAdd(&stats[1], n_dirty);
