@@ -59,12 +59,15 @@ def median(a):
     return float(np.median(a))
 
 
-def run_gpu(A, x, dtype, cfg, warmup, steps, flush_bytes):
+def run_gpu(A, x, dtype, cfg, warmup, steps, flush_bytes, memory=0):
     """Times the workload with the learned tuner's tile (or `cfg` when given),
-    then the fixed default tile and the exhaustive autotuner's best."""
+    then the fixed default tile and the exhaustive autotuner's best.  memory:
+    0 = device memory, 1 = managed memory, 2 = host-mapped memory (the last two
+    are USE_SVM_ALWAYS=1, spmv_cuda/Makefile:6)."""
     from benchmark_spmv_using_csr_amd import tuner
 
     op = bhsparse_spmv(dtype)
+    _lib.check(op.set_memory_mode(memory), "set_memory_mode")
     _lib.check(op.init_platform(0), "init_platform")
     y = np.zeros(A.m, dtype)
     t0 = time.perf_counter()
@@ -93,7 +96,7 @@ def run_gpu(A, x, dtype, cfg, warmup, steps, flush_bytes):
     wt, wm = op.bench(1.0, warmup=3, reps=reps2, flush_bytes=0)
     fl = op.bench_floor(reps=reps2, flush_bytes=flush_bytes)
     ov = op.bench_event_overhead(reps=reps2, flush_bytes=flush_bytes)
-    # the method without the tuner, and the exhaustive judge
+    # the method without the tuner, and the exhaustive autotuner
     _lib.check(op.set_config(*tuner.CLASSES[tuner.DEFAULT_CLASS]), "set_config")
     dt, _ = op.bench(1.0, warmup=3, reps=reps2, flush_bytes=flush_bytes)
     us, best = op.autotune(tuner.CLASSES, reps=9, flush_bytes=flush_bytes)
@@ -216,6 +219,10 @@ def main():
     ap.add_argument("--mtx", default="",
                     help="Matrix Market file to use instead of a synthetic workload "
                          "(sorted, duplicates merged; x uniform in [-1, 1])")
+    ap.add_argument("--memory", choices=["device", "managed", "host-mapped"], default="device",
+                    help="where A, x and y live: hipMalloc (default), hipMallocManaged, or "
+                         "pinned host memory mapped into the device (the last two are the "
+                         "reference's USE_SVM_ALWAYS=1 build)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {a.gpus} is not supported "
@@ -241,7 +248,9 @@ def main():
         A = matgen.block3_2m()
         x = matgen.vector(104, A.n)
 
-    r = run_gpu(A, x, dtype, cfg, a.warmup, a.steps, a.flush_mb << 20)
+    managed = a.memory != "device"
+    r = run_gpu(A, x, dtype, cfg, a.warmup, a.steps, a.flush_mb << 20,
+                {"device": 0, "managed": 1, "host-mapped": 2}[a.memory])
     cfg = r["cfg"]
     vs = np.dtype(dtype).itemsize
     b = eff_bytes(A.m, A.n, A.nnz, vs)
@@ -255,7 +264,7 @@ def main():
     value = b / (med_total * 1e-3) / 1e9
     achieved = b / (mean_main * 1e-3) / 1e9
     S = cfg[1]
-    prof = "" if a.mtx else f"segsum_{a.workload.replace('-', '')}_{a.dtype}_traffic.json"
+    prof = "" if (a.mtx or managed) else f"segsum_{a.workload.replace('-', '')}_{a.dtype}_traffic.json"
     traffic = load_traffic(prof, cfg)
     floor_ms = max(median(r["floor"]) - ev_ms, 1e-6)
     copy_ms = stream_copy(512 << 20, reps=10)
@@ -278,7 +287,8 @@ def main():
         "vs_baseline": None,
         "dtype": a.dtype,
         "data": ("Matrix Market file" if a.mtx else "synthetic (seeded generator)") +
-                ", inputs resident in HBM, L2+MALL flushed per step",
+                (f", A, x and y in {a.memory} memory" if managed else
+                 ", inputs resident in HBM") + ", L2+MALL flushed per step",
         "config": {
             "workload": A.name, "m": A.m, "n": A.n, "nnz": A.nnz,
             "effective_bytes": b,
@@ -286,6 +296,7 @@ def main():
                        2: "speculative segmented sum over column panels"}[cfg[0]],
             "tile": tuner.class_name(cfg), "tile_source": r["cfg_source"],
             "parallelism": "single GPU",
+            "memory": a.memory,
         },
         "pct_hbm_peak": round(100 * value / HBM_PEAK_GBPS, 2),
         "roofline": {

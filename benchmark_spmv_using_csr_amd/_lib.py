@@ -78,6 +78,8 @@ _PER_TYPE = {
     "init_platform": [_H, C.c_int],
     "init_data": [_H, C.c_int, C.c_int, C.c_int, _i32p, _i32p, "T", "T", C.c_void_p],
     "set_x": [_H, "T"],
+    "set_memory_mode": [_H, C.c_int],
+    "get_memory_mode": [_H, C.POINTER(C.c_int)],
     "spmv": [_H, "V"],
     "spmv_async": [_H, "V"],
     "get_y": [_H],

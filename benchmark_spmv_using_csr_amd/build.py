@@ -76,14 +76,17 @@ def build_host() -> str:
 
 
 def build_example() -> str:
-    """The C++ drop-in example (reference's `./spmv 0`) against the built library."""
-    out = os.path.join(PKG, "examples", "spmv_small")
-    tmp = out + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}",
-          os.path.join(PKG, "examples", "spmv_small.cpp"), f"-L{PKG}", "-lbhsparse_spmv",
-          f"-Wl,-rpath,{PKG}", "-o", tmp])
-    os.replace(tmp, out)
-    return out
+    """The C++ drop-in example (reference's `./spmv 0`) against the built
+    library, as the default build and as the unified-memory build
+    (-DUSE_SVM_ALWAYS=1, the reference's spmv_cuda/Makefile:6 switch)."""
+    for name, defs in (("spmv_small", []), ("spmv_small_svm", ["-DUSE_SVM_ALWAYS=1"])):
+        out = os.path.join(PKG, "examples", name)
+        tmp = out + ".tmp"
+        _run(["g++", "-O2", "-std=c++17", *defs, f"-I{os.path.join(ROOT, 'include')}",
+              os.path.join(PKG, "examples", "spmv_small.cpp"), f"-L{PKG}", "-lbhsparse_spmv",
+              f"-Wl,-rpath,{PKG}", "-o", tmp])
+        os.replace(tmp, out)
+    return os.path.join(PKG, "examples", "spmv_small")
 
 
 def build_all() -> None:

@@ -4,8 +4,9 @@
 //
 // Reference counterparts: bhsparse_spmv_cuda (spmv_cuda/bhsparse_spmv_cuda.h)
 // and bhsparse_spmv_opencl (spmv_opencl_amd/bhsparse_spmv_opencl.h).  Unlike
-// them, every buffer is plain device memory (hipMalloc), nothing is computed
-// on the host per call, and a call never depends on y's previous contents.
+// them, every buffer is plain device memory (hipMalloc) unless the handle is
+// put in managed mode, nothing is computed on the host per call, and a call
+// never depends on y's previous contents.
 #include <hip/hip_ext.h>
 
 #include "bhs_kernels.h"
@@ -65,15 +66,22 @@ struct Impl {
     int *d_row_ptr = nullptr, *d_col = nullptr;
     T *d_val = nullptr, *d_x = nullptr, *d_y = nullptr;
     T *h_y = nullptr;   // borrowed, as _h_y in bhsparse_spmv_cuda.h:248
+    // BHS_MEMORY_MANAGED / BHS_MEMORY_HOST_MAPPED: row_ptr, col, val, x and y
+    // come from hipMallocManaged / pinned host memory mapped into the device
+    // (the USE_SVM_ALWAYS=1 build, bhsparse_spmv_cuda.h:250-263); data_mode is
+    // the mode the loaded data was allocated in
+    int mem_mode = BHS_MEMORY_DEVICE, data_mode = BHS_MEMORY_DEVICE;
 
     // the fixed default tile (the tuner's "no tuner" baseline class)
     bhs_config cfg{BHS_VARIANT_SEGSUM, 8, 4, 4, 0};
 
     // column-panel copy of A (BHS_VARIANT_SEGSUM_PANELS): one stacked row per
-    // nonempty (panel, row) pair, panel-major; pan_idx[p*m+i] = its stacked row
-    // or -1; pan_n = 0 when not built
+    // nonempty (panel, row) pair, panel-major.  d_pmask[i] bit p: row i has
+    // entries in panel p; d_pbase[g*pan_n+p]: stacked rows before (panel p,
+    // row 64g).  pan_n = 0 when not built
     int pan_n = 0, pan_rows = 0;
-    int *d_prow_ptr = nullptr, *d_pcol = nullptr, *d_pidx = nullptr;
+    int *d_prow_ptr = nullptr, *d_pcol = nullptr, *d_pbase = nullptr;
+    unsigned char *d_pmask = nullptr;
     T *d_pval = nullptr, *d_ystack = nullptr;
 
     // analysis for the current tile shape and layout (an_layout = panels, 0 = A)
@@ -170,16 +178,29 @@ int init_platform(bhs_handle_t h, int device)
     return BHSPARSE_SUCCESS;
 }
 
+// Frees an array allocated by alloc_data in `mode`.
+template <typename U>
+void free_data(U *&p, int mode)
+{
+    if (p) {
+        if (mode == BHS_MEMORY_HOST_MAPPED)
+            (void)hipHostFree((void *)p);
+        else
+            (void)hipFree((void *)p);
+    }
+    p = nullptr;
+}
+
 template <typename T>
 void release_data(Impl<T> *im)
 {
     if (im->platform) (void)hipSetDevice(im->device);
     if (im->stream) (void)hipStreamSynchronize(im->stream);
-    free_ptr(im->d_row_ptr);
-    free_ptr(im->d_col);
-    free_ptr(im->d_val);
-    free_ptr(im->d_x);
-    free_ptr(im->d_y);
+    free_data(im->d_row_ptr, im->data_mode);
+    free_data(im->d_col, im->data_mode);
+    free_data(im->d_val, im->data_mode);
+    free_data(im->d_x, im->data_mode);
+    free_data(im->d_y, im->data_mode);
     free_ptr(im->d_tile_row);
     free_ptr(im->d_sync_row);
     free_ptr(im->d_sync_val);
@@ -188,7 +209,8 @@ void release_data(Impl<T> *im)
     free_ptr(im->d_pcol);
     free_ptr(im->d_pval);
     free_ptr(im->d_ystack);
-    free_ptr(im->d_pidx);
+    free_ptr(im->d_pmask);
+    free_ptr(im->d_pbase);
     im->pan_n = im->pan_rows = 0;
     im->floor_cap = 0;
     im->tile_cap = im->sync_cap = 0;
@@ -197,7 +219,31 @@ void release_data(Impl<T> *im)
     im->NP = im->NW = im->lead_empty = 0;
     im->h_y = nullptr;
     im->data = false;
+    im->data_mode = BHS_MEMORY_DEVICE;
     im->m = im->n = im->nnz = 0;
+}
+
+// Allocation of the caller-visible arrays (A, x, y) in the data's memory mode
+// (data_mode).  Host-mapped memory must be usable by kernels at the address
+// the host sees (one pointer for both, as in unified memory).
+template <typename T, typename U>
+hipError_t alloc_data(Impl<T> *im, U **p, size_t bytes)
+{
+    if (im->data_mode == BHS_MEMORY_MANAGED)
+        return hipMallocManaged(reinterpret_cast<void **>(p), bytes, hipMemAttachGlobal);
+    if (im->data_mode == BHS_MEMORY_HOST_MAPPED) {
+        hipError_t e = hipHostMalloc(reinterpret_cast<void **>(p), bytes, hipHostMallocMapped);
+        if (e != hipSuccess) return e;
+        void *dp = nullptr;
+        e = hipHostGetDevicePointer(&dp, (void *)*p, 0);
+        if (e == hipSuccess && dp != (void *)*p) e = hipErrorInvalidValue;
+        if (e != hipSuccess) {
+            (void)hipHostFree((void *)*p);
+            *p = nullptr;
+        }
+        return e;
+    }
+    return hipMalloc(reinterpret_cast<void **>(p), bytes);
 }
 
 inline bool is_segsum(int variant)
@@ -253,7 +299,8 @@ int build_panels(Impl<T> *im, int npan)
     free_ptr(im->d_pcol);
     free_ptr(im->d_pval);
     free_ptr(im->d_ystack);
-    free_ptr(im->d_pidx);
+    free_ptr(im->d_pmask);
+    free_ptr(im->d_pbase);
     im->pan_n = im->pan_rows = 0;
     const int m = im->m, n = im->n, nnz = im->nnz;
     if ((long long)npan * m > 0x7fffffffLL)
@@ -285,6 +332,26 @@ int build_panels(Impl<T> *im, int npan)
         }
     }
     const int M = (int)prp.size() - 1;
+    // combine metadata: per-row panel bits, per-(64-row group, panel) base
+    const long long G = ((long long)m + 63) / 64;
+    std::vector<unsigned char> mask((size_t)m, 0);
+    std::vector<int> base((size_t)(G * npan));
+    for (int p = 0; p < npan; ++p) {
+        int next = -1;   // first stacked row of panel p at or after the group
+        for (long long g = G - 1; g >= 0; --g) {
+            for (long long i = std::min<long long>(m, (g + 1) * 64) - 1; i >= g * 64; --i) {
+                const int k = idx[(size_t)p * m + (size_t)i];
+                if (k >= 0) {
+                    next = k;
+                    mask[(size_t)i] |= (unsigned char)(1u << p);
+                }
+            }
+            base[(size_t)(g * npan + p)] = next;
+        }
+    }
+    // a group with no row in panel p never reads its base; give it any value
+    for (auto &b : base)
+        if (b < 0) b = 0;
     std::vector<int> pos(prp.begin(), prp.end() - 1), pcol((size_t)nnz);
     std::vector<T> pval((size_t)nnz);
     for (int i = 0; i < m; ++i)
@@ -297,11 +364,14 @@ int build_panels(Impl<T> *im, int npan)
     BHS_HIP(hipMalloc(&im->d_pcol, sizeof(int) * (size_t)im->nnz_pad));
     BHS_HIP(hipMalloc(&im->d_pval, sizeof(T) * (size_t)im->nnz_pad));
     BHS_HIP(hipMalloc(&im->d_ystack, sizeof(T) * (size_t)std::max(M, 1)));
-    BHS_HIP(hipMalloc(&im->d_pidx, sizeof(int) * std::max<size_t>(PM, 1)));
+    BHS_HIP(hipMalloc(&im->d_pmask, std::max<size_t>((size_t)m, 1)));
+    BHS_HIP(hipMalloc(&im->d_pbase, sizeof(int) * std::max<size_t>(base.size(), 1)));
     BHS_HIP(hipMemcpyAsync(im->d_prow_ptr, prp.data(), sizeof(int) * ((size_t)M + 1),
                            hipMemcpyHostToDevice, im->stream));
-    BHS_HIP(hipMemcpyAsync(im->d_pidx, idx.data(), sizeof(int) * PM, hipMemcpyHostToDevice,
+    BHS_HIP(hipMemcpyAsync(im->d_pmask, mask.data(), (size_t)m, hipMemcpyHostToDevice,
                            im->stream));
+    BHS_HIP(hipMemcpyAsync(im->d_pbase, base.data(), sizeof(int) * base.size(),
+                           hipMemcpyHostToDevice, im->stream));
     BHS_HIP(hipMemsetAsync(im->d_pcol, 0, sizeof(int) * (size_t)im->nnz_pad, im->stream));
     BHS_HIP(hipMemsetAsync(im->d_pval, 0, sizeof(T) * (size_t)im->nnz_pad, im->stream));
     if (nnz > 0) {
@@ -383,15 +453,28 @@ int init_data(bhs_handle_t h, int m, int n, int nnz, const int *row_ptr, const i
 
     release_data(im);
     BHS_HIP(hipSetDevice(im->device));
+    if (im->mem_mode == BHS_MEMORY_MANAGED) {
+        int managed = 0;
+        BHS_HIP(hipDeviceGetAttribute(&managed, hipDeviceAttributeManagedMemory, im->device));
+        if (!managed)
+            return set_error(BHSPARSE_UNSUPPORTED_DEVICE,
+                             "device has no managed memory (hipDeviceAttributeManagedMemory = 0)");
+    } else if (im->mem_mode == BHS_MEMORY_HOST_MAPPED) {
+        int can_map = 0;
+        BHS_HIP(hipDeviceGetAttribute(&can_map, hipDeviceAttributeCanMapHostMemory, im->device));
+        if (!can_map)
+            return set_error(BHSPARSE_UNSUPPORTED_DEVICE, "device cannot map host memory");
+    }
     im->m = m;
     im->n = n;
     im->nnz = nnz;
     im->nnz_pad = ((long long)nnz + kMaxTile - 1) / kMaxTile * kMaxTile + kMaxTile;
-    BHS_HIP(hipMalloc(&im->d_row_ptr, sizeof(int) * ((size_t)m + 1)));
-    BHS_HIP(hipMalloc(&im->d_col, sizeof(int) * (size_t)im->nnz_pad));
-    BHS_HIP(hipMalloc(&im->d_val, sizeof(T) * (size_t)im->nnz_pad));
-    BHS_HIP(hipMalloc(&im->d_x, sizeof(T) * (size_t)std::max(n, 1)));
-    BHS_HIP(hipMalloc(&im->d_y, sizeof(T) * (size_t)std::max(m, 1)));
+    im->data_mode = im->mem_mode;
+    BHS_HIP(alloc_data(im, &im->d_row_ptr, sizeof(int) * ((size_t)m + 1)));
+    BHS_HIP(alloc_data(im, &im->d_col, sizeof(int) * (size_t)im->nnz_pad));
+    BHS_HIP(alloc_data(im, &im->d_val, sizeof(T) * (size_t)im->nnz_pad));
+    BHS_HIP(alloc_data(im, &im->d_x, sizeof(T) * (size_t)std::max(n, 1)));
+    BHS_HIP(alloc_data(im, &im->d_y, sizeof(T) * (size_t)std::max(m, 1)));
     BHS_HIP(hipMemcpyAsync(im->d_row_ptr, row_ptr, sizeof(int) * ((size_t)m + 1),
                            hipMemcpyHostToDevice, im->stream));
     BHS_HIP(hipMemsetAsync(im->d_col, 0, sizeof(int) * (size_t)im->nnz_pad, im->stream));
@@ -424,6 +507,35 @@ int set_x(bhs_handle_t h, const T *x)
         BHS_HIP(hipMemcpyAsync(im->d_x, x, sizeof(T) * (size_t)im->n, hipMemcpyHostToDevice,
                                im->stream));
     BHS_HIP(hipStreamSynchronize(im->stream));
+    return BHSPARSE_SUCCESS;
+}
+
+// Memory mode for the next init_data (the reference's compile-time
+// USE_SVM_ALWAYS switch, spmv_cuda/Makefile:6).  Changing it while data is loaded is an
+// error: free_mem first.
+template <typename T>
+int set_memory_mode(bhs_handle_t h, int mode)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (mode != BHS_MEMORY_DEVICE && mode != BHS_MEMORY_MANAGED &&
+        mode != BHS_MEMORY_HOST_MAPPED)
+        return set_error(BHSPARSE_INVALID_ARGUMENT,
+                         "memory mode must be device (0), managed (1) or host-mapped (2)");
+    if (im->data && mode != im->data_mode)
+        return set_error(BHSPARSE_INVALID_ARGUMENT,
+                         "data is loaded in the other memory mode; free_mem first");
+    im->mem_mode = mode;
+    return BHSPARSE_SUCCESS;
+}
+
+template <typename T>
+int get_memory_mode(bhs_handle_t h, int *mode)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (!mode) return set_error(BHSPARSE_INVALID_ARGUMENT, "null mode");
+    *mode = im->data ? im->data_mode : im->mem_mode;
     return BHSPARSE_SUCCESS;
 }
 
@@ -480,10 +592,11 @@ template <typename T>
 void launch_combine(Impl<T> *im)
 {
     if (im->cfg.variant != BHS_VARIANT_SEGSUM_PANELS) return;
-    const int threads = 256;
-    const long long blocks = std::min<long long>(((long long)im->m + threads - 1) / threads, 8192);
+    const int threads = 256;   // one wave per 64 rows
+    const long long blocks = ((long long)im->m + threads - 1) / threads;
     hipLaunchKernelGGL(bhs::combine_panels_kernel<T>, dim3((unsigned)blocks), dim3(threads), 0,
-                       im->stream, im->d_ystack, im->d_pidx, im->m, im->pan_n, im->d_y);
+                       im->stream, im->d_ystack, im->d_pmask, im->d_pbase, im->m, im->pan_n,
+                       im->d_y);
 }
 
 // TPV rule of main.cu:238-287 / csr_vector_opencl.h:212-230 (by floor(nnz/m)).
@@ -1113,6 +1226,8 @@ int bhs_stream_copy(int device, long long bytes, int reps, double *ms)
         return init_data<T>(h, m, n, nnz, rp, ci, v, x, y);                                    \
     }                                                                                          \
     int bhs_##P##_set_x(bhs_handle_t h, const T *x) { return set_x<T>(h, x); }                \
+    int bhs_##P##_set_memory_mode(bhs_handle_t h, int md) { return set_memory_mode<T>(h, md); } \
+    int bhs_##P##_get_memory_mode(bhs_handle_t h, int *md) { return get_memory_mode<T>(h, md); } \
     int bhs_##P##_spmv(bhs_handle_t h, T a) { return spmv<T>(h, a); }                         \
     int bhs_##P##_spmv_async(bhs_handle_t h, T a) { return spmv_async<T>(h, a); }             \
     int bhs_##P##_get_y(bhs_handle_t h) { return get_y<T>(h); }                               \

@@ -697,24 +697,33 @@ __global__ __launch_bounds__(256) void calibrate_batched_kernel(const int *__res
         calibrate_window<T, KMAX>(sync_row, sync_val, sync_k, NW, base, rr, lane, y);
 }
 
-// y[i] = sum over panels p (in order) of row i's panel partial: ystack[idx[p*m+i]],
-// +0 where row i has no entry in panel p.  The last step of the column-panel
-// variant; every partial already carries alpha.
+// y[i] = sum over panels p (in order) of row i's panel partial, +0 where row i
+// has no entry in panel p.  The last step of the column-panel variant; every
+// partial already carries alpha.  One wave per group g of 64 rows: bit p of
+// mask[i] says row i has entries in panel p, and that partial is stacked row
+// base[g*npan+p] + (rows of the group below i with bit p set), a ballot and a
+// popcount.  Reads m bytes of mask instead of an npan x m index map.
 template <typename T>
 __global__ __launch_bounds__(256) void combine_panels_kernel(const T *__restrict__ ystack,
-                                                            const int *__restrict__ idx, int m,
+                                                            const unsigned char *__restrict__ mask,
+                                                            const int *__restrict__ base, int m,
                                                             int npan, T *__restrict__ y)
 {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m;
-         i += (long long)gridDim.x * blockDim.x) {
-        int k = idx[i];
-        T s = k >= 0 ? ystack[k] : T(0);
-        for (int p = 1; p < npan; ++p) {
-            k = idx[(long long)p * m + i];
-            s += k >= 0 ? ystack[k] : T(0);
-        }
-        y[i] = s;
+    const int lane = lane_id();
+    const long long g = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (g * kWave >= m) return;                       // whole wave past the last row
+    const long long i = g * kWave + lane;
+    const unsigned mk = i < m ? (unsigned)mask[i] : 0u;
+    const int *gb = base + g * npan;
+    T s = T(0);
+    for (int p = 0; p < npan; ++p) {
+        const bool has = (mk >> p) & 1u;
+        const uint64_t bits = ballot(has);             // every lane of the wave is active
+        const int pos = has ? gb[p] + popc_below(bits) : 0;
+        const T v = has ? ystack[pos] : T(0);
+        s = p == 0 ? v : s + v;
     }
+    if (i < m) y[i] = s;
 }
 
 template <typename T>

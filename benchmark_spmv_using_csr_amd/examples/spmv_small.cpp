@@ -3,6 +3,7 @@
 // 0, 12, 42].  Unlike the reference (which never calls get_y, main.cu:499-531)
 // it reads y back, and runs the SpMV twice to show the result is repeatable.
 // Build: g++ -std=c++17 -I include spmv_small.cpp -L <pkg> -lbhsparse_spmv
+// (add -DUSE_SVM_ALWAYS=1 for the unified-memory build, as in spmv_cuda/Makefile:6).
 #include <cstdio>
 
 #include "bhsparse_spmv.hpp"
@@ -31,6 +32,11 @@ int main()
         if (!err) err = bhsparse->get_y();
         for (int i = 0; i < 6; ++i) bad += y[i] != expect[i];
     }
+    int mode = -1;
+    bhsparse->getMemoryMode(&mode);
+    std::printf("memory: %s\n", mode == BHS_MEMORY_MANAGED       ? "managed"
+                                 : mode == BHS_MEMORY_HOST_MAPPED ? "host-mapped"
+                                                                  : "device");
     std::printf("y = [%g, %g, %g, %g, %g, %g] %s\n", (double)y[0], (double)y[1], (double)y[2],
                 (double)y[3], (double)y[4], (double)y[5], (!err && !bad) ? "PASS" : "FAIL");
     bhsparse->free_platform();

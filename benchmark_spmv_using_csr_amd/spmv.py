@@ -3,7 +3,7 @@
 ``bhsparse_spmv`` keeps the method names and meaning of ``bhsparse_spmv_cuda``
 (spmv_cuda/bhsparse_spmv_cuda.h:7-55): ``init_platform`` / ``prepare_mem`` /
 ``run_benchmark`` / ``get_y`` / ``free_platform`` / ``free_mem``, plus the
-north-star names ``initPlatform`` / ``initData`` / ``spmv``.  Every call goes
+camel-case names ``initPlatform`` / ``initData`` / ``spmv``.  Every call goes
 through the C ABI of ``libbhsparse_spmv.so`` (gfx950 HIP kernels).  Integer
 return codes are kept (0 = BHSPARSE_SUCCESS) as in the reference; ``check``
 turns them into exceptions where a caller prefers that.
@@ -16,6 +16,11 @@ import numpy as np
 
 from . import _lib
 from ._lib import Config, SpmvStats, check, spmv_lib
+
+
+MEMORY_DEVICE = 0        # BHS_MEMORY_DEVICE
+MEMORY_MANAGED = 1       # BHS_MEMORY_MANAGED (the reference's USE_SVM_ALWAYS=1 build)
+MEMORY_HOST_MAPPED = 2   # BHS_MEMORY_HOST_MAPPED (pinned host memory mapped into the device)
 
 
 class bhsparse_spmv:  # noqa: N801  (reference class name)
@@ -75,7 +80,7 @@ class bhsparse_spmv:  # noqa: N801  (reference class name)
         self._keep = None
         return rc
 
-    # -- north-star names ---------------------------------------------------
+    # -- camel-case names ---------------------------------------------------
     def initPlatform(self, device: int = 0) -> int:  # noqa: N802
         return self.init_platform(device)
 
@@ -94,6 +99,16 @@ class bhsparse_spmv:  # noqa: N801  (reference class name)
         if len(xx) == 0:
             xx = np.zeros(1, self.dtype)
         return self._fn("set_x")(self._h, xx)
+
+    def set_memory_mode(self, mode) -> int:
+        """MEMORY_DEVICE, MEMORY_MANAGED or MEMORY_HOST_MAPPED for the next
+        prepare_mem (spmv_cuda/Makefile:6, bhsparse_spmv_cuda.h:250-263)."""
+        return self._fn("set_memory_mode")(self._h, int(mode))
+
+    def memory_mode(self) -> int:
+        md = C.c_int(-1)
+        check(self._fn("get_memory_mode")(self._h, C.byref(md)), "get_memory_mode")
+        return md.value
 
     def y(self) -> np.ndarray:
         out = np.empty(max(self.m, 1), self.dtype)

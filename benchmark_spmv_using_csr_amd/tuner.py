@@ -7,7 +7,7 @@ segmented sum at a tile shape (items per lane x waves per block x tiles per
 wave), the same over a column-panel copy of A, or csr-vector at a width --
 from 8 row-length statistics that the GPU reduces from row_ptr/col.  The network predicts log2(time / best time) for
 every class; the tuner takes the argmin.  It is trained on timing sweeps this
-module generates (the exhaustive autotuner, bhs_*_autotune, which also judges
+module generates (the exhaustive autotuner, bhs_*_autotune, which also measures
 the tuner's regret), by the library's own HIP forward/backward/Adam kernels.
 
 Command line (GPU):

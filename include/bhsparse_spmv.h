@@ -44,6 +44,20 @@ extern "C" {
  * count from the device copy of A (host pass, O(nnz)) and kept with the handle. */
 #define BHS_VARIANT_SEGSUM_PANELS 2
 
+/* Where init_data puts row_ptr, col, val, x and y (bhs_*_set_memory_mode).
+ * DEVICE: hipMalloc (default).  MANAGED: hipMallocManaged, the analogue of the
+ * reference's USE_SVM_ALWAYS=1 build (spmv_cuda/Makefile:6,
+ * bhsparse_spmv_cuda.h:250-263), where A and x live in unified memory; needs a
+ * device with managed memory (hipDeviceAttributeManagedMemory), else
+ * init_data returns BHSPARSE_UNSUPPORTED_DEVICE.  HOST_MAPPED: pinned host
+ * memory mapped into the device (hipHostMalloc), one pointer valid on host and
+ * device, which every device that can map host memory supports; kernels read
+ * A and x over the host link.  The tile partition, calibration buffers and
+ * panel copy stay in device memory in every mode. */
+#define BHS_MEMORY_DEVICE      0
+#define BHS_MEMORY_MANAGED     1
+#define BHS_MEMORY_HOST_MAPPED 2
+
 typedef struct bhs_handle_opaque *bhs_handle_t;
 /* The tuner's network (a device-resident MLP); see "learned tuner" below. */
 typedef struct bhs_mlp_opaque *bhs_mlp_t;
@@ -117,6 +131,13 @@ int bhs_s_init_data(bhs_handle_t h, int m, int n, int nnz, const int *row_ptr,
 /* Replace x (n values) without re-uploading A. No reference counterpart
  * (the reference re-runs prepare_mem). */
 int bhs_s_set_x(bhs_handle_t h, const float *x);
+/* Memory mode used by the next init_data (USE_SVM_ALWAYS, a compile-time
+ * switch in the reference, spmv_cuda/Makefile:6).  Rejected while data loaded in the
+ * other mode is held (free_mem first).  BHSPARSE_UNSUPPORTED_DEVICE from
+ * init_data if the device has no managed memory. */
+int bhs_s_set_memory_mode(bhs_handle_t h, int mode);
+/* Mode of the loaded data, or the mode the next init_data will use. */
+int bhs_s_get_memory_mode(bhs_handle_t h, int *mode);
 /* run_benchmark() (bhsparse_spmv_cuda.h:121-228), plus alpha, which the
  * reference drops (utils_cuda.h:133).  Blocking, like the reference. */
 int bhs_s_spmv(bhs_handle_t h, float alpha);
@@ -183,6 +204,8 @@ int bhs_d_init_platform(bhs_handle_t h, int device);
 int bhs_d_init_data(bhs_handle_t h, int m, int n, int nnz, const int *row_ptr,
                     const int *col, const double *val, const double *x, double *y);
 int bhs_d_set_x(bhs_handle_t h, const double *x);
+int bhs_d_set_memory_mode(bhs_handle_t h, int mode);
+int bhs_d_get_memory_mode(bhs_handle_t h, int *mode);
 int bhs_d_spmv(bhs_handle_t h, double alpha);
 int bhs_d_spmv_async(bhs_handle_t h, double alpha);
 int bhs_d_get_y(bhs_handle_t h);

@@ -1,6 +1,6 @@
 // bhsparse_spmv.hpp -- header-only C++ operator class over the C ABI of
 // bhsparse_spmv.h, with the method names of the reference's
-// bhsparse_spmv_cuda (spmv_cuda/bhsparse_spmv_cuda.h:7-55) and the north-star
+// bhsparse_spmv_cuda (spmv_cuda/bhsparse_spmv_cuda.h:7-55) and the camel-case
 // names beside them.  value_type follows USE_DOUBLE exactly as in
 // spmv_cuda/common.h:27-31 (default: double).
 //
@@ -41,6 +41,8 @@ template <> struct abi<float> {
     static int free_platform(bhs_handle_t h) { return bhs_s_free_platform(h); }
     static int destroy(bhs_handle_t h) { return bhs_s_destroy(h); }
     static int set_config(bhs_handle_t h, const bhs_config *c) { return bhs_s_set_config(h, c); }
+    static int set_memory_mode(bhs_handle_t h, int md) { return bhs_s_set_memory_mode(h, md); }
+    static int get_memory_mode(bhs_handle_t h, int *md) { return bhs_s_get_memory_mode(h, md); }
 };
 
 template <> struct abi<double> {
@@ -58,12 +60,21 @@ template <> struct abi<double> {
     static int free_platform(bhs_handle_t h) { return bhs_d_free_platform(h); }
     static int destroy(bhs_handle_t h) { return bhs_d_destroy(h); }
     static int set_config(bhs_handle_t h, const bhs_config *c) { return bhs_d_set_config(h, c); }
+    static int set_memory_mode(bhs_handle_t h, int md) { return bhs_d_set_memory_mode(h, md); }
+    static int get_memory_mode(bhs_handle_t h, int *md) { return bhs_d_get_memory_mode(h, md); }
 };
 
 template <typename T>
 class spmv_operator {
 public:
-    spmv_operator() { status_ = abi<T>::create(&h_); }
+    spmv_operator()
+    {
+        status_ = abi<T>::create(&h_);
+#if defined(USE_SVM_ALWAYS) && USE_SVM_ALWAYS
+        // the reference's unified-memory build (spmv_cuda/Makefile:6)
+        if (!status_) status_ = abi<T>::set_memory_mode(h_, BHS_MEMORY_MANAGED);
+#endif
+    }
     ~spmv_operator()
     {
         if (h_) abi<T>::destroy(h_);
@@ -71,15 +82,28 @@ public:
     spmv_operator(const spmv_operator &) = delete;
     spmv_operator &operator=(const spmv_operator &) = delete;
 
-    // north-star names
+    // camel-case names
     int initPlatform(int device = 0) { return status_ ? status_ : abi<T>::init_platform(h_, device); }
     int initData(int m, int n, int nnzA, int *csrRowPtrA, int *csrColIdxA, T *csrValA, T *x, T *y)
     {
-        return abi<T>::init_data(h_, m, n, nnzA, csrRowPtrA, csrColIdxA, csrValA, x, y);
+        int rc = abi<T>::init_data(h_, m, n, nnzA, csrRowPtrA, csrColIdxA, csrValA, x, y);
+#if defined(USE_SVM_ALWAYS) && USE_SVM_ALWAYS
+        // a device without managed memory: pinned host memory mapped into the
+        // device is the unified memory it has (one pointer on host and device)
+        int mode = -1;
+        if (rc == BHSPARSE_UNSUPPORTED_DEVICE && !abi<T>::get_memory_mode(h_, &mode) &&
+            mode == BHS_MEMORY_MANAGED &&
+            !abi<T>::set_memory_mode(h_, BHS_MEMORY_HOST_MAPPED))
+            rc = abi<T>::init_data(h_, m, n, nnzA, csrRowPtrA, csrColIdxA, csrValA, x, y);
+#endif
+        return rc;
     }
     int spmv(T alpha = T(1)) { return abi<T>::spmv(h_, alpha); }
     int setX(const T *x) { return abi<T>::set_x(h_, x); }
     int setConfig(const bhs_config &c) { return abi<T>::set_config(h_, &c); }
+    // BHS_MEMORY_DEVICE, _MANAGED or _HOST_MAPPED, before initData
+    int setMemoryMode(int mode) { return abi<T>::set_memory_mode(h_, mode); }
+    int getMemoryMode(int *mode) { return abi<T>::get_memory_mode(h_, mode); }
 
     // reference names (bhsparse_spmv_cuda.h:11-17)
     int init_platform() { return initPlatform(0); }

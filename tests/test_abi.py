@@ -79,6 +79,18 @@ def test_host_side_argument_checks():
     assert L.bhs_d_set_config(h, C.byref(good)) == 0
     out = _lib.Config()
     assert L.bhs_d_get_config(h, C.byref(out)) == 0 and out.as_tuple() == (0, 16, 2, 5, 0)
+    # memory mode (USE_SVM_ALWAYS, spmv_cuda/Makefile:6): 0 / 1 / 2, device by default
+    md = C.c_int(-1)
+    assert L.bhs_d_get_memory_mode(h, C.byref(md)) == 0 and md.value == 0
+    assert L.bhs_d_set_memory_mode(h, 3) == _lib.BHSPARSE_INVALID_ARGUMENT
+    assert L.bhs_d_set_memory_mode(h, -1) == _lib.BHSPARSE_INVALID_ARGUMENT
+    assert L.bhs_d_set_memory_mode(h, 2) == 0
+    assert L.bhs_d_get_memory_mode(h, C.byref(md)) == 0 and md.value == 2
+    assert L.bhs_d_set_memory_mode(h, 1) == 0
+    assert L.bhs_d_get_memory_mode(h, C.byref(md)) == 0 and md.value == 1
+    assert L.bhs_d_get_memory_mode(h, None) == _lib.BHSPARSE_INVALID_ARGUMENT
+    assert L.bhs_s_set_memory_mode(h, 1) == _lib.BHSPARSE_UNSUPPORTED_VALUE_TYPE
+    assert L.bhs_d_set_memory_mode(h, 0) == 0
     assert L.bhs_d_free_mem(h) == 0 and L.bhs_d_free_mem(h) == 0     # twice is fine
     assert L.bhs_d_destroy(h) == 0
     assert L.bhs_d_spmv(None, 1.0) == _lib.BHSPARSE_INVALID_ARGUMENT

@@ -49,3 +49,29 @@ def test_model_alpha():
     y1 = km.segsum(c.row_ptr, c.col, c.val, c.x, S=8, TPW=3, alpha=1.0)
     y2 = km.segsum(c.row_ptr, c.col, c.val, c.x, S=8, TPW=3, alpha=-0.25)
     assert np.array_equal(y2, -0.25 * y1)
+
+
+@pytest.mark.parametrize("npan", [2, 4, 8])
+def test_panel_combine_positions(npan):
+    """The combine step's addressing (one wave per 64 rows: bit p of a per-row
+    mask, a per-(group, panel) base, and the rank of the lane among the
+    group's rows with bit p) finds exactly the stacked row of every nonempty
+    (panel, row) pair, and nothing past the stacked matrix."""
+    for c in cases.adversarial(512, 3, seed=npan):
+        st, idx = cases.stack_panels(c, npan)
+        m = c.m
+        G = (m + 63) // 64
+        mask = np.zeros(m, np.int64)
+        for p in range(npan):
+            mask |= (idx[p] >= 0).astype(np.int64) << p
+        for p in range(npan):
+            for g in range(G):
+                rows = np.arange(64 * g, min(m, 64 * g + 64))
+                has = (mask[rows] >> p) & 1 == 1
+                if not has.any():
+                    continue
+                base = idx[p][rows[has]].min()          # first stacked row of the group
+                rank = np.cumsum(has) - has              # lower lanes with the bit
+                pos = base + rank[has]
+                assert np.array_equal(pos, idx[p][rows[has]]), (c.name, p, g)
+                assert pos.max() < st.m
