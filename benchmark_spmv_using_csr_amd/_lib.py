@@ -45,7 +45,8 @@ class Config(C.Structure):
 
 class SpmvStats(C.Structure):
     _fields_ = [(n, C.c_longlong) for n in
-                ("n_tiles", "n_waves", "n_fast", "n_dirty", "n_sync", "n_lead_empty")]
+                ("n_tiles", "n_waves", "n_fast", "n_dirty", "n_sync", "n_lead_empty",
+                 "chunk_rows")]
 
 
 class BatchSchedule(C.Structure):

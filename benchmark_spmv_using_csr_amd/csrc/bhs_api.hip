@@ -87,6 +87,7 @@ struct Impl {
     // analysis for the current tile shape and layout (an_layout = panels, 0 = A)
     int an_P = 0, an_TPW = 0, an_layout = -1;
     int NP = 0, NW = 0, lead_empty = 0;
+    int chunk_parts = 2;   // row chunk of the segmented sum: 2 or 4 parts of 64 rows
     int *d_tile_row = nullptr;
     int tile_cap = 0;
     int *d_sync_row = nullptr;
@@ -418,12 +419,26 @@ int ensure_analysis(Impl<T> *im)
         hipLaunchKernelGGL(bhs::partition_kernel, dim3(blocks), dim3(threads), 0, im->stream,
                            L.row_ptr, L.rows, im->nnz, P, (int)NP, im->d_tile_row);
     BHS_HIP(hipGetLastError());
+    // tiles spanning 128-255 rows: with enough of them (5% of the tiles) the
+    // 256-row chunk spares their row_ptr loop, otherwise the 128-row chunk
+    // costs less per tile
+    unsigned long long mid_tiles = 0;
+    if (L.rows > 0 && NP > 0) {
+        launch_fill(im, im->d_counters, 1);
+        hipLaunchKernelGGL(bhs::tile_span_count_kernel, dim3((unsigned)((NP + threads - 1) / threads)),
+                           dim3(threads), 0, im->stream, (const int *)im->d_tile_row, (int)NP,
+                           2 * 64, 4 * 64, im->d_counters);
+        BHS_HIP(hipGetLastError());
+        BHS_HIP(hipMemcpyAsync(&mid_tiles, im->d_counters, sizeof(mid_tiles),
+                               hipMemcpyDeviceToHost, im->stream));
+    }
     int first = 0;
     BHS_HIP(hipMemcpyAsync(&first, im->d_tile_row, sizeof(int), hipMemcpyDeviceToHost, im->stream));
     BHS_HIP(hipStreamSynchronize(im->stream));
     im->NP = (int)NP;
     im->NW = (int)NW;
     im->lead_empty = L.rows > 0 ? first : 0;
+    im->chunk_parts = (NP > 0 && mid_tiles * 20 >= (unsigned long long)NP) ? 4 : 2;
     im->an_P = P;
     im->an_TPW = TPW;
     im->an_layout = layout;
@@ -539,7 +554,7 @@ int get_memory_mode(bhs_handle_t h, int *mode)
     return BHSPARSE_SUCCESS;
 }
 
-template <typename T, int S, bool STATS>
+template <typename T, int S, bool STATS, int CP>
 void launch_segsum_s(Impl<T> *im, T alpha, unsigned long long *stats, hipEvent_t e0, hipEvent_t e1)
 {
     const Layout<T> L = active_layout(im);
@@ -556,26 +571,36 @@ void launch_segsum_s(Impl<T> *im, T alpha, unsigned long long *stats, hipEvent_t
     const int blocks = (im->NW + q.wpb - 1) / q.wpb;
     const size_t lds = (size_t)q.wpb * q.wave_lds_bytes;
     if (e0 && e1)
-        hipExtLaunchKernelGGL((bhs::segsum_kernel<T, S, STATS>), dim3(blocks), dim3(64 * q.wpb),
+        hipExtLaunchKernelGGL((bhs::segsum_kernel<T, S, STATS, CP>), dim3(blocks), dim3(64 * q.wpb),
                               (std::uint32_t)lds, im->stream, e0, e1, 0u, L.row_ptr, L.col, L.val,
                               (const T *)im->d_x, L.y, (const int *)im->d_tile_row,
                               im->d_sync_row, im->d_sync_val, stats, q);
     else
-        hipLaunchKernelGGL((bhs::segsum_kernel<T, S, STATS>), dim3(blocks), dim3(64 * q.wpb), lds,
+        hipLaunchKernelGGL((bhs::segsum_kernel<T, S, STATS, CP>), dim3(blocks), dim3(64 * q.wpb), lds,
                            im->stream, L.row_ptr, L.col, L.val, im->d_x, L.y, im->d_tile_row,
                            im->d_sync_row, im->d_sync_val, stats, q);
 }
 
+template <typename T, bool STATS, int CP>
+void launch_segsum_cp(Impl<T> *im, T alpha, unsigned long long *stats, hipEvent_t e0,
+                      hipEvent_t e1)
+{
+    switch (im->cfg.items_per_lane) {
+    case 4: launch_segsum_s<T, 4, STATS, CP>(im, alpha, stats, e0, e1); break;
+    case 8: launch_segsum_s<T, 8, STATS, CP>(im, alpha, stats, e0, e1); break;
+    case 16: launch_segsum_s<T, 16, STATS, CP>(im, alpha, stats, e0, e1); break;
+    default: launch_segsum_s<T, 32, STATS, CP>(im, alpha, stats, e0, e1); break;
+    }
+}
+
+// The row chunk covers 128 rows (2 parts) or 256 (4 parts), as the analysis
+// of the tile partition chose.
 template <typename T, bool STATS>
 void launch_segsum(Impl<T> *im, T alpha, unsigned long long *stats, hipEvent_t e0 = nullptr,
                    hipEvent_t e1 = nullptr)
 {
-    switch (im->cfg.items_per_lane) {
-    case 4: launch_segsum_s<T, 4, STATS>(im, alpha, stats, e0, e1); break;
-    case 8: launch_segsum_s<T, 8, STATS>(im, alpha, stats, e0, e1); break;
-    case 16: launch_segsum_s<T, 16, STATS>(im, alpha, stats, e0, e1); break;
-    default: launch_segsum_s<T, 32, STATS>(im, alpha, stats, e0, e1); break;
-    }
+    if (im->chunk_parts == 4) launch_segsum_cp<T, STATS, 4>(im, alpha, stats, e0, e1);
+    else launch_segsum_cp<T, STATS, 2>(im, alpha, stats, e0, e1);
 }
 
 template <typename T>
@@ -846,6 +871,7 @@ int get_stats(bhs_handle_t h, bhs_spmv_stats *out)
     out->n_tiles = im->NP;
     out->n_waves = im->NW;
     out->n_lead_empty = im->lead_empty;
+    out->chunk_rows = is_segsum(im->cfg.variant) ? 64LL * im->chunk_parts : 0;
     if (im->NP == 0 || im->m == 0) return BHSPARSE_SUCCESS;
     if (segsum_lds_bytes<T>(im->cfg) > im->max_lds)
         return set_error(BHSPARSE_INVALID_ARGUMENT,

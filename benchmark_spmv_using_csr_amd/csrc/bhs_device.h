@@ -9,9 +9,27 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+#include <utility>
+
 namespace bhs {
 
 constexpr int kWave = 64;
+
+// f(std::integral_constant<int, 0>{}), ..., f(std::integral_constant<int, N-1>{}):
+// a loop whose index is a constant expression in the body, so small register
+// arrays indexed by it are never placed in scratch memory.
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_seq(F &&f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
 
 __device__ __forceinline__ int lane_id()
 {
@@ -68,6 +86,8 @@ enum : int {
     kDppRowHalfMirror = 0x141,// lane i <- lane 7-i inside each half row of 8
     kDppRowBcast15 = 0x142,   // row r+1 <- lane 15 of row r
     kDppRowBcast31 = 0x143,   // rows 2, 3 <- lane 31
+    kDppWaveShl1 = 0x130,     // lane i <- lane i+1 across the wave (lane 63 keeps `old`)
+    kDppWaveShr1 = 0x138,     // lane i <- lane i-1 across the wave (lane 0 keeps `old`)
 };
 
 template <int CTRL, int ROW_MASK = 0xF>
@@ -131,7 +151,8 @@ __device__ __forceinline__ T wave_segscan(T v, uint64_t head, int lane)
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) {
         if (cov == ~0ull) break;
-        T up = __shfl_up(v, d, kWave);
+        // lane i <- lane i-d (lane 0 is a head and never uses its value)
+        T up = d == 1 ? dpp<kDppWaveShr1>(v) : __shfl_up(v, d, kWave);
         if (!((cov >> lane) & 1ull)) v = up + v;
         cov |= cov << d;
     }

@@ -92,20 +92,56 @@ __device__ __forceinline__ void tile_load(const int *__restrict__ col, const T *
     }
 }
 
-// Row boundaries of the first 128 rows of a tile starting at row r: lane l holds
-// row_ptr[r+l], row_ptr[r+l+1], row_ptr[r+64+l], row_ptr[r+64+l+1], with the
-// row index clamped to m-1 (every address in bounds; callers ignore rows past
-// the tile's last row).  Loaded one tile ahead of its use.
+// Row boundaries of the first 64*CP rows of a tile starting at row r (CP = 2
+// or 4 chunk parts): lane l holds a[q] = row_ptr[r + 64q + l] for q < CP, and
+// every lane holds end = row_ptr[r + 64*CP], each index clamped to m (row_ptr
+// has m+1 entries, so every address is in bounds; callers ignore rows past the
+// tile's last row).  A row's end, row_ptr[r + 64q + l + 1], is lane l+1's a[q]
+// (lane 63: a[q+1] of lane 0, or end), so CP loads per lane cover 64*CP rows.
+// Loaded one tile ahead of its use.  CP = 4 pays on matrices where many tiles
+// span 128-255 rows (short rows), CP = 2 elsewhere; the host picks per matrix.
+template <int CP>
 struct RowChunk {
-    int a0, b0, a1, b1;
+    int a[CP];
+    int end;
 };
 
-__device__ __forceinline__ RowChunk load_chunk(const int *__restrict__ row_ptr, int r, int lane,
-                                               int m)
+template <int CP>
+__device__ __forceinline__ RowChunk<CP> load_chunk(const int *__restrict__ row_ptr, int r,
+                                                   int lane, int m)
 {
-    const int i0 = min(r + lane, m - 1);
-    const int i1 = min(r + kWave + lane, m - 1);
-    return RowChunk{row_ptr[i0], row_ptr[i0 + 1], row_ptr[i1], row_ptr[i1 + 1]};
+    RowChunk<CP> c;
+    static_for<CP>([&](auto qc) { c.a[qc] = row_ptr[min(r + qc * kWave + lane, m)]; });
+    c.end = row_ptr[min(r + CP * kWave, m)];
+    return c;
+}
+
+// row_ptr[r + 64Q + lane + 1] from a chunk loaded at row r.  Reads other
+// lanes: call it with every lane of the wave active.
+template <int Q, int CP>
+__device__ __forceinline__ int chunk_next(const RowChunk<CP> &c, int lane)
+{
+    int first_after;
+    if constexpr (Q + 1 < CP) first_after = read_lane(c.a[Q + 1], 0);
+    else first_after = c.end;
+    const int up = dpp<kDppWaveShl1>(c.a[Q], first_after);
+    return lane == kWave - 1 ? first_after : up;
+}
+
+// Number of tiles p < NP whose row span tile_row[p+1] - tile_row[p] lies in
+// [lo, hi), added to *count (one atomic per wave).
+__global__ __launch_bounds__(256) void tile_span_count_kernel(const int *__restrict__ tile_row,
+                                                              int NP, int lo, int hi,
+                                                              unsigned long long *count)
+{
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    bool in = false;
+    if (p < NP) {
+        const int d = tile_row[p + 1] - tile_row[p];
+        in = d >= lo && d < hi;
+    }
+    const uint64_t b = ballot(in);
+    if (lane_id() == 0 && b) atomicAdd(count, (unsigned long long)__popcll(b));
 }
 
 // Compile-time switch between the tile code paths of segsum_wave.
@@ -143,9 +179,9 @@ __device__ __forceinline__ int empty_run_end(const int *__restrict__ row_ptr, in
 // Per tile [B, B+P) with r0 = tile_row[p], r1 = tile_row[p+1]:
 //  * products val*x[col] (kernels_cuda.h:150-162): coalesced loads, the x
 //    gathers all issued before any of them is waited on; the next tile's
-//    col/val stream and the row_ptr values of its first 128 rows are in
+//    col/val stream and the row_ptr values of its first 256 rows are in
 //    flight while this tile is reduced, so no tile waits on a row_ptr load
-//    unless it spans more than 128 rows.
+//    unless it spans more than 256 rows.
 //  * fast track (r0 == r1, the tile lies inside one row, :79-112): lane sums,
 //    wave sum, added to the carried partial; nothing is stored until the row
 //    closes.
@@ -193,7 +229,7 @@ struct MatView {
     int m, nnz, NP, NW, lead_empty, k, kstride, TPW, key_base;
 };
 
-template <typename T, int S, int KMAX, bool STRIDED, bool STATS>
+template <typename T, int S, int KMAX, bool STRIDED, bool STATS, int CP>
 __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, const int lane,
                                             unsigned *s_head, T *s_buf, T *s_carry, const T alpha,
                                             unsigned long long *stats)
@@ -250,7 +286,8 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
     // is in flight while this one is reduced (the last tile reloads itself).
     Tile<T, S> cur;
     tile_load<T, S>(mv.col, mv.val, E0, lane, cur);
-    RowChunk rc = load_chunk(row_ptr, r_first, lane, mv.m);
+    constexpr int CR = CP * kWave;              // rows a chunk covers
+    RowChunk<CP> rc = load_chunk<CP>(row_ptr, r_first, lane, mv.m);
     int r1 = r_first;
 
     for (int p = p_begin; p < p_end; ++p) {
@@ -261,13 +298,13 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
         T xv[S];
 #pragma unroll
         for (int kk = 0; kk < S; ++kk) xv[kk] = x[(long long)cur.c[kk] * ks];
-        const RowChunk cc = rc;
-        rc = load_chunk(row_ptr, r1, lane, mv.m);
+        const RowChunk<CP> cc = rc;
+        rc = load_chunk<CP>(row_ptr, r1, lane, mv.m);
         __builtin_amdgcn_sched_barrier(0);
         Tile<T, S> nxt;
         tile_load<T, S>(mv.col, mv.val, (p + 1 < p_end) ? B + P : B, lane, nxt);
 
-        const bool cont = read_lane(cc.a0, 0) < B;   // item 0 continues row r0 (lane 0 holds row r0)
+        const bool cont = read_lane(cc.a[0], 0) < B;   // item 0 continues row r0 (lane 0 holds row r0)
         if (p == p_begin) syncing = cont;
         const bool tail = B + P > mv.nnz;
 
@@ -292,34 +329,41 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
             open = true;
             if (STATS) ++n_fast;
         } else {
-            // A tile of at most 128 rows takes every row boundary from its
-            // chunk; only longer tiles load more row_ptr values, in loops.
-            // The two cases are separate instantiations, so the loops (and
-            // the waits they need) are not on the common path at all.
+            // A tile of fewer than CR rows takes every row boundary from
+            // its chunk; only longer tiles load more row_ptr values, in
+            // loops.  The two cases are separate instantiations, so the loops
+            // (and the waits they need) are not on the common path at all.
             auto general_tile = [&](auto big) __attribute__((always_inline)) {
                 constexpr bool BIG = decltype(big)::value;
                 // ---- bit flags (once for all right-hand sides) ------------------
                 const int stop = (r1 == mv.m) ? mv.m - 1 : r1;
-                const bool have0 = r0 + lane <= stop;
-                const bool have1 = r0 + kWave + lane <= stop;
-                const bool ne0 = have0 && cc.b0 > cc.a0;
-                const bool ne1 = have1 && cc.b1 > cc.a1;
-                bool lane_dirty = (have0 && !ne0) || (have1 && !ne1);
+                // nonempty rows of the chunk, one ballot per 64 rows (kept as
+                // wave-wide masks, not per-lane registers, until the output)
+                uint64_t mne[CP];
+                bool lane_dirty = false;
                 wave_sync();                         // previous tile's LDS reads are done
                 s_head[lane] = 0u;
                 wave_sync();
                 // (empty rows of the chunk get their zero in the output step,
                 // after this tile's loads have been consumed)
-                if (ne0) {
-                    const int o = cc.a0 - B;
-                    if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
-                }
-                if (ne1) {
-                    const int o = cc.a1 - B;
-                    if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
-                }
+                static_for<CP>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    mne[q] = 0ull;
+                    if (q > 0 && r0 + q * kWave > stop) return;   // wave-uniform: no rows here
+                    // the shift reads the next lane: it runs on the whole
+                    // wave, before any lane-dependent condition masks lanes off
+                    const int next = chunk_next<q>(cc, lane);
+                    const bool have = r0 + q * kWave + lane <= stop;
+                    const bool ne = have && next > cc.a[q];
+                    lane_dirty = lane_dirty || (have && !ne);
+                    mne[q] = ballot(ne);
+                    if (ne) {
+                        const int o = cc.a[q] - B;
+                        if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
+                    }
+                });
                 if constexpr (BIG)
-                for (int rb = r0 + 2 * kWave; rb <= stop;) {   // > 128 rows
+                for (int rb = r0 + CR; rb <= stop;) {   // >= CR rows
                     const int rq = min(rb + lane, stop);
                     const int rp = row_ptr[rq], rpn = row_ptr[rq + 1];
                     const int vb = row_ptr[min(rb + kWave, stop + 1)];
@@ -353,18 +397,27 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                 const int incl = wave_incl_scan(nh, lane);
                 const int seg_base = incl - nh;
                 const int K = read_lane(incl, kWave - 1);
-                // row_ptr[r1] sits in the chunk when r1 < r0+128
+                // row_ptr[r1] sits in the chunk when r1 < r0+CR
                 int rp_r1;
-                if (r1 - r0 < kWave) rp_r1 = read_lane(cc.a0, r1 - r0);
-                else if (!BIG || r1 - r0 < 2 * kWave) rp_r1 = read_lane(cc.a1, r1 - r0 - kWave);
-                else rp_r1 = row_ptr[min(r1, mv.m)];
+                if (!BIG || r1 - r0 < CR) {
+                    // read the lane out of every part, then pick one: a
+                    // select among registers by a run-time index would be
+                    // compiled as an indexed load from scratch memory
+                    const int d = r1 - r0, q = d / kWave, l = d % kWave;
+                    rp_r1 = read_lane(cc.a[0], l);
+                    static_for<CP>([&](auto qc) {
+                        if constexpr (decltype(qc)::value > 0) {
+                            const int v = read_lane(cc.a[qc], l);
+                            if (q == qc) rp_r1 = v;
+                        }
+                    });
+                } else {
+                    rp_r1 = row_ptr[min(r1, mv.m)];
+                }
                 const bool partial_tail = (r1 < mv.m) && (rp_r1 < B + P);
                 const int kb = syncing ? 1 : 0;
                 const int ke = partial_tail ? K - 1 : K;
-                const uint64_t m0 = ballot(ne0);
-                const uint64_t m1 = ballot(ne1);
-                const int rank0 = popc_below(m0);
-                const int rank1 = __popcll(m0) + popc_below(m1);
+
                 if (STATS && dirty) ++n_dirty;
 
                 for (int rr = 0; rr < k; ++rr) {
@@ -410,7 +463,7 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                     // ---- segments spanning lanes -----------------------------
                     const uint64_t present = ballot(seen);  // bit 0 always set
                     const T I = wave_segscan(acc, present, lane);
-                    const T Iprev = __shfl_up(I, 1, kWave);
+                    const T Iprev = dpp<kDppWaveShr1>(I);     // lane i <- lane i-1 (lane 0: unused)
                     if (seen && lane > 0) s_buf[seg_base - 1] = Iprev + lead;
                     if (lane == kWave - 1) s_buf[K - 1] = I;
                     wave_sync();
@@ -420,17 +473,23 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                         for (int kk = kb + lane; kk < ke; kk += kWave)
                             y[(long long)(r0 + kk) * ks + rr] = alpha * s_buf[kk];
                     } else {
-                        if (ne0 && rank0 >= kb && rank0 < ke)
-                            y[(long long)(r0 + lane) * ks + rr] = alpha * s_buf[rank0];
-                        else if (have0 && !ne0)
-                            y[(long long)(r0 + lane) * ks + rr] = T(0);
-                        if (ne1 && rank1 >= kb && rank1 < ke)
-                            y[(long long)(r0 + kWave + lane) * ks + rr] = alpha * s_buf[rank1];
-                        else if (have1 && !ne1)
-                            y[(long long)(r0 + kWave + lane) * ks + rr] = T(0);
-                        int running = __popcll(m0) + __popcll(m1);
+                        // a nonempty row's segment is its rank among the
+                        // tile's nonempty rows; an empty row gets 0
+                        int running = 0;
+                        static_for<CP>([&](auto qc) {
+                            constexpr int q = decltype(qc)::value;
+                            if (q > 0 && r0 + q * kWave > stop) return;   // wave-uniform
+                            const int row = r0 + q * kWave + lane;
+                            const bool ne = (mne[q] >> lane) & 1ull;
+                            const int rank = running + popc_below(mne[q]);
+                            if (ne && rank >= kb && rank < ke)
+                                y[(long long)row * ks + rr] = alpha * s_buf[rank];
+                            else if (row <= stop && !ne)
+                                y[(long long)row * ks + rr] = T(0);
+                            running += __popcll(mne[q]);
+                        });
                         if constexpr (BIG)
-                        for (int rb = r0 + 2 * kWave; rb <= stop;) {   // > 128 rows
+                        for (int rb = r0 + CR; rb <= stop;) {   // >= CR rows
                             const int rq = min(rb + lane, stop);
                             const int rp = row_ptr[rq], rpn = row_ptr[rq + 1];
                             const int vb = row_ptr[min(rb + kWave, stop + 1)];
@@ -465,7 +524,7 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                 // tile (which would otherwise wait for every load in flight)
                 if constexpr (BIG) __builtin_amdgcn_s_waitcnt(0);
             };
-            if (r1 - r0 < 2 * kWave) general_tile(BigTag<false>{});
+            if (r1 - r0 < CR) general_tile(BigTag<false>{});
             else general_tile(BigTag<true>{});
         }
         cur = nxt;
@@ -495,7 +554,7 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
     }
 }
 
-template <typename T, int S, bool STATS>
+template <typename T, int S, bool STATS, int CP>
 __global__ __launch_bounds__(512) void segsum_kernel(
     const int *__restrict__ row_ptr, const int *__restrict__ col, const T *__restrict__ val,
     const T *__restrict__ x, T *__restrict__ y, const int *__restrict__ tile_row,
@@ -511,7 +570,7 @@ __global__ __launch_bounds__(512) void segsum_kernel(
     unsigned char *wave_lds = reinterpret_cast<unsigned char *>(smem_raw) + wib * q.wave_lds_bytes;
     MatView<T> mv{row_ptr, tile_row, col, val, x, y, sync_row, sync_val,
                   q.m, q.nnz, q.NP, q.NW, q.lead_empty, 1, 1, q.TPW, 0};
-    segsum_wave<T, S, 1, false, STATS>(mv, w, lane, reinterpret_cast<unsigned *>(wave_lds),
+    segsum_wave<T, S, 1, false, STATS, CP>(mv, w, lane, reinterpret_cast<unsigned *>(wave_lds),
                                        reinterpret_cast<T *>(wave_lds + 256), nullptr, q.alpha,
                                        stats);
 }
@@ -571,11 +630,11 @@ __device__ __forceinline__ void calibrate_window(const int *__restrict__ sync_ro
     const int r = key(e);
     const T v = r >= 0 ? sync_val[(long long)e * KMAX + rr] : T(0);
     const int before = key(base - 1);
-    int rprev = __shfl_up(r, 1, kWave);
+    int rprev = dpp<kDppWaveShr1>(r);                 // lane i <- lane i-1
     if (lane == 0) rprev = before;
     const uint64_t starts = ballot(r != rprev) | 1ull;
     const T s = wave_segscan(v, starts, lane);
-    const int rnext = __shfl_down(r, 1, kWave);
+    const int rnext = dpp<kDppWaveShl1>(r);           // lane i <- lane i+1 (lane 63: unused)
     const bool run_end = (lane == kWave - 1) || (rnext != r);
     // the run holding lane 0 continues an earlier window's run?
     const int r0w = read_lane(r, 0);
@@ -666,7 +725,7 @@ __global__ __launch_bounds__(512) void batched_kernel(
                       sync_row + md.sync_off, sync_val + (long long)md.sync_off * KMAX,
                       md.m, md.nnz, md.NP, md.NW, md.lead_empty, md.k, md.k, TPW,
                       (int)md.y_off};
-        segsum_wave<T, S, KMAX, true, false>(mv, iw.y, lane, s_head, s_buf, s_carry, alpha,
+        segsum_wave<T, S, KMAX, true, false, 2>(mv, iw.y, lane, s_head, s_buf, s_carry, alpha,
                                              nullptr);
         if (lane == 0) sync_k[md.sync_off + iw.y] = md.k;
         wave_sync();                              // LDS is reused by the next item

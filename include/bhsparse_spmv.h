@@ -71,6 +71,8 @@ typedef struct {
     long long n_dirty;       /* tiles holding an empty row (speculation fixed up) */
     long long n_sync;        /* waves that sent a boundary-row partial to calibration */
     long long n_lead_empty;  /* empty rows before the first nonzero */
+    long long chunk_rows;    /* rows whose row_ptr each tile loads ahead: 128, or 256 when
+                                at least 5% of the tiles span 128-255 rows (0: csr-vector) */
 } bhs_spmv_stats;
 
 /* Tile shape / variant.  S = items per lane (4, 8, 16, 32), waves_per_block

@@ -98,10 +98,22 @@ def adversarial(P, T, seed=0, integer=False):
     add("empty rows at tile boundary", [P // 2, 0, 0, P // 2, 0, P, 0, 1])
     add("dirty last tile", [5] * (P // 5) + [1, 0, 2, 0, 0])
     add("huge empty run", [3] + [0] * 5000 + [4] + [0] * 300 + [2])
-    # runs of empty rows around the 64- and 128-row chunk sizes, a run far
-    # longer than a tile inside it, and one reaching the last row
+    # runs of empty rows around 64-row multiples (the chunk parts and the
+    # 256-row chunk), a run far longer than a tile inside it, and one reaching
+    # the last row
     add("empty runs of 63/64/65/127/128/129", [2] + [0] * 63 + [3] + [0] * 64 + [1] + [0] * 65 +
         [2] + [0] * 127 + [1] + [0] * 128 + [3] + [0] * 129 + [2])
+    add("empty runs of 191/192/193/255/256/257", [1] + [0] * 191 + [2] + [0] * 192 + [1] +
+        [0] * 193 + [3] + [0] * 255 + [1] + [0] * 256 + [2] + [0] * 257 + [1])
+    # tiles of 128-256 and 256+ short rows with a few empty ones (the chunk
+    # covers 256 rows; longer tiles walk the rest)
+    add("short rows with sparse empties", ([1, 1, 2, 0] * 300 + [1] * 100)[:1300])
+    # about 190 rows per tile (so the 256-row chunk is chosen), 5% of them empty
+    L = P / 190.0
+    nr = 190 * (3 * T + 2)
+    lens = np.floor(L) + (rng.random(nr) < L - np.floor(L))
+    lens[rng.random(nr) < 0.05] = 0
+    add("about 190 rows per tile", lens.astype(np.int64))
     add("100k empty rows inside a tile", [5] * 40 + [0] * 100000 + [3] * 30 + [0] * 700 + [1] * 9,
         n=512)
     add("empty run to the last row", [4] * 20 + [0] * 20000, n=256)

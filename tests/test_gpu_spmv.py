@@ -108,6 +108,32 @@ def test_counters_match_oracle(cfg):
     for k in ("n_tiles", "n_waves", "n_fast", "n_dirty", "n_sync", "n_lead_empty"):
         assert st[k] == ts[k], k
     assert st["n_fast"] > 0 and st["n_dirty"] > 0 and st["n_sync"] > 0
+    assert st["chunk_rows"] == expected_chunk_rows(c, S)
+
+
+def expected_chunk_rows(c, S):
+    """256 when at least 5% of the tiles span 128-255 rows, else 128."""
+    tr = oracle.partition(c.row_ptr, c.nnz, 64 * S)
+    span = np.diff(np.asarray(tr, np.int64))
+    return 256 if 20 * int(((span >= 128) & (span < 256)).sum()) >= len(span) else 128
+
+
+@pytest.mark.parametrize("S", [4, 8, 16])
+def test_row_chunk_choice(S):
+    """A matrix of about 190 rows per tile takes the 256-row chunk, one of
+    about 30 the 128-row chunk; both give the model's bits."""
+    rng = np.random.default_rng(S)
+    for per_tile in (190, 30):
+        Lm = 64 * S / per_tile
+        lens = (np.floor(Lm) + (rng.random(20000) < Lm - np.floor(Lm))).astype(np.int64)
+        lens[rng.random(20000) < 0.05] = 0
+        c = cases.from_lengths(f"{per_tile} rows per tile", lens, 3000, seed=S)
+        say(f"row chunk S={S} rows/tile~{per_tile}")
+        y, op = gpu_spmv(c, np.float64, (S, 4, 3))
+        want = expected_chunk_rows(c, S)
+        assert want == (256 if per_tile == 190 else 128)
+        assert op.stats()["chunk_rows"] == want
+        assert np.array_equal(y, km.segsum(c.row_ptr, c.col, c.val, c.x, S=S, TPW=3))
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
