@@ -30,6 +30,9 @@ from benchmark_spmv_using_csr_amd.spmv import (bhsparse_spmv, device_synchronize
                                                stream_copy)
 
 HBM_PEAK_GBPS = 8000.0
+# the tile comes from the learned tuner's shortlist: its TUNER_TOPK best-predicted
+# classes are timed and the fastest is kept
+TUNER_TOPK = 3
 METRIC = "effective GB/s (% HBM peak) CSR SpMV; tuned+batched vs plain speedup at equal err"
 
 
@@ -74,13 +77,17 @@ def run_gpu(A, x, dtype, cfg, warmup, steps, flush_bytes, memory=0):
     _lib.check(op.prepare_mem(A.m, A.n, A.nnz, A.row_ptr, A.col, A.val.astype(dtype),
                               x.astype(dtype), y), "prepare_mem")
     t_init = time.perf_counter() - t0
+    top1 = None
     if cfg is None:
         net = tuner.make_net(tuner.load_net())
+        top1 = op.apply_tuner(net, tuner.CLASSES)          # the network's own pick
         t0 = time.perf_counter()
-        chosen = op.apply_tuner(net, tuner.CLASSES)
+        chosen = op.apply_tuner(net, tuner.CLASSES, topk=TUNER_TOPK, reps=7,
+                                flush_bytes=flush_bytes)
         t_tune = time.perf_counter() - t0
         cfg = tuner.CLASSES[chosen]
-        source = "learned tuner (data/tuner_net.json)"
+        source = (f"learned tuner (data/tuner_net.json): its {TUNER_TOPK} best-predicted "
+                  "classes timed, fastest kept")
     else:
         _lib.check(op.set_config(*cfg), "set_config")
         t_tune = 0.0
@@ -102,7 +109,7 @@ def run_gpu(A, x, dtype, cfg, warmup, steps, flush_bytes, memory=0):
     us, best = op.autotune(tuner.CLASSES, reps=9, flush_bytes=flush_bytes)
     op.close()
     return {
-        "cfg": cfg, "cfg_source": source, "tune_s": t_tune, "chosen": chosen,
+        "cfg": cfg, "cfg_source": source, "tune_s": t_tune, "chosen": chosen, "top1": top1,
         "ms_total": ms_total, "ms_main": ms_main, "wall_s": wall, "init_s": t_init,
         "warm_total": wt, "warm_main": wm, "floor": fl, "event_overhead": ov,
         "default_ms": median(dt),
@@ -318,7 +325,13 @@ def main():
             },
         },
         "tuner": {
-            "chosen": tuner.class_name(cfg), "inference_s": round(r["tune_s"], 5),
+            "chosen": tuner.class_name(cfg), "tuning_s": round(r["tune_s"], 5),
+            "shortlist": TUNER_TOPK,
+            "network_top1": (tuner.class_name(tuner.CLASSES[r["top1"]])
+                             if r["top1"] is not None else None),
+            "network_top1_regret_vs_exhaustive": (
+                round(float(us[r["top1"]]) / float(us[best]) - 1, 4)
+                if r["top1"] is not None else None),
             "default_tile": tuner.class_name(tuner.CLASSES[tuner.DEFAULT_CLASS]),
             "default_ms": round(r["default_ms"], 5),
             "tuned_speedup_vs_default": (

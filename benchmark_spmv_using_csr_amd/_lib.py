@@ -99,6 +99,8 @@ _PER_TYPE = {
     "autotune": [_H, C.POINTER(Config), C.c_int, C.c_int, C.c_longlong, C.c_void_p,
                  C.POINTER(C.c_int)],
     "apply_tuner": [_H, C.c_void_p, C.POINTER(Config), C.c_int, C.POINTER(C.c_int)],
+    "apply_tuner_topk": [_H, C.c_void_p, C.POINTER(Config), C.c_int, C.c_int, C.c_int,
+                         C.c_longlong, C.POINTER(C.c_int)],
 }
 
 _BATCH = {

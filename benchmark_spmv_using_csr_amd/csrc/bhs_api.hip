@@ -98,6 +98,7 @@ struct Impl {
     T *d_floor = nullptr;                       // per-lane partials of the floor probe
     long long floor_cap = 0;
     int *d_tuner_io = nullptr;                  // tuner inference inputs/outputs (ints)
+    float *d_tuner_scores = nullptr;            // tuner's predicted cost per class
     void *d_flush = nullptr;
     long long flush_cap = 0;
     // ev[0]/ev[2]: recorded around a whole call; ev[1]/ev[3]: attached to the
@@ -796,6 +797,7 @@ int free_platform(bhs_handle_t h)
     im->flush_cap = 0;
     free_ptr(im->d_counters);
     free_ptr(im->d_tuner_io);
+    free_ptr(im->d_tuner_scores);
     for (auto &e : im->ev)
         if (e) { (void)hipEventDestroy(e); e = nullptr; }
     if (im->stream) (void)hipStreamDestroy(im->stream);
@@ -1082,13 +1084,14 @@ int autotune(bhs_handle_t h, const bhs_config *cands, int n, int reps, long long
     return ensure_analysis(im);
 }
 
-// Learned choice: row statistics (GPU reduction) -> features -> MLP -> argmin,
-// all on the device; installs classes[*chosen].
+// Learned prediction: row statistics (GPU reduction) -> features -> MLP, all
+// on the device.  *choice = argmin of the predicted cost over the classes that
+// can run; if scores is given, it receives every class's predicted cost and
+// runnable[k] whether class k can run on this handle.
 template <typename T>
-int apply_tuner(bhs_handle_t h, bhs_mlp_t net, const bhs_config *classes, int K, int *chosen)
+int tuner_predict(Impl<T> *im, bhs_mlp_t net, const bhs_config *classes, int K, int *choice,
+                  std::vector<float> *scores, std::vector<int> *runnable)
 {
-    Impl<T> *im;
-    if (int rc = get_impl(h, &im)) return rc;
     if (!im->data) return set_error(BHSPARSE_NOT_INITIALIZED, "init_data first");
     if (!net || !classes) return set_error(BHSPARSE_INVALID_ARGUMENT, "null network/classes");
     const bhs::Mlp *mlp = reinterpret_cast<const bhs::Mlp *>(net);
@@ -1109,6 +1112,8 @@ int apply_tuner(bhs_handle_t h, bhs_mlp_t net, const bhs_config *classes, int K,
         io[4 + k] = ok;
     }
     if (!im->d_tuner_io) BHS_HIP(hipMalloc(&im->d_tuner_io, sizeof(int) * (4 + bhs::kMaxClasses)));
+    if (scores && !im->d_tuner_scores)
+        BHS_HIP(hipMalloc(&im->d_tuner_scores, sizeof(float) * bhs::kMaxClasses));
     BHS_HIP(hipMemcpyAsync(im->d_tuner_io, io.data(), sizeof(int) * io.size(),
                            hipMemcpyHostToDevice, im->stream));
     launch_fill(im, im->d_counters, 8);
@@ -1118,14 +1123,64 @@ int apply_tuner(bhs_handle_t h, bhs_mlp_t net, const bhs_config *classes, int K,
                            im->d_row_ptr, im->d_col, im->d_counters);
     }
     BHS_HIP(bhs::launch_tuner_infer(im->d_counters, im->d_tuner_io, im->d_tuner_io + 2, 1, *mlp, K,
-                                    im->d_tuner_io + 4, im->d_tuner_io + 3, nullptr, im->stream));
+                                    im->d_tuner_io + 4, im->d_tuner_io + 3,
+                                    scores ? im->d_tuner_scores : nullptr, im->stream));
     int c = -1;
     BHS_HIP(hipMemcpyAsync(&c, im->d_tuner_io + 3, sizeof(int), hipMemcpyDeviceToHost, im->stream));
+    if (scores) {
+        scores->resize(K);
+        BHS_HIP(hipMemcpyAsync(scores->data(), im->d_tuner_scores, sizeof(float) * K,
+                               hipMemcpyDeviceToHost, im->stream));
+    }
     BHS_HIP(hipStreamSynchronize(im->stream));
     if (c < 0 || c >= K) return set_error(BHSPARSE_INVALID_ARGUMENT, "no runnable class");
+    if (runnable) runnable->assign(io.begin() + 4, io.end());
+    *choice = c;
+    return BHSPARSE_SUCCESS;
+}
+
+// Learned choice: installs the class with the lowest predicted cost.
+template <typename T>
+int apply_tuner(bhs_handle_t h, bhs_mlp_t net, const bhs_config *classes, int K, int *chosen)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    int c = -1;
+    if (int rc = tuner_predict(im, net, classes, K, &c, nullptr, nullptr)) return rc;
     im->cfg = classes[c];
     if (chosen) *chosen = c;
     return ensure_analysis(im);
+}
+
+// Learned shortlist: the topk runnable classes with the lowest predicted cost
+// are timed as the exhaustive autotuner times them, and the fastest is
+// installed.  topk = 1 is apply_tuner.
+template <typename T>
+int apply_tuner_topk(bhs_handle_t h, bhs_mlp_t net, const bhs_config *classes, int K, int topk,
+                     int reps, long long flush_bytes, int *chosen)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (topk < 1 || reps < 1)
+        return set_error(BHSPARSE_INVALID_ARGUMENT, "topk and reps must be at least 1");
+    if (topk == 1) return apply_tuner<T>(h, net, classes, K, chosen);
+    int c = -1;
+    std::vector<float> sc;
+    std::vector<int> ok;
+    if (int rc = tuner_predict(im, net, classes, K, &c, &sc, &ok)) return rc;
+    std::vector<int> order;
+    for (int k = 0; k < K; ++k)
+        if (ok[k]) order.push_back(k);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sc[a] < sc[b]; });
+    if ((int)order.size() > topk) order.resize(topk);
+    std::vector<bhs_config> cands;
+    for (int k : order) cands.push_back(classes[k]);
+    std::vector<double> us(cands.size());
+    int bi = -1;
+    if (int rc = autotune<T>(h, cands.data(), (int)cands.size(), reps, flush_bytes, us.data(), &bi))
+        return rc;
+    if (chosen) *chosen = order[bi];
+    return BHSPARSE_SUCCESS;
 }
 
 template <typename T>
@@ -1291,6 +1346,11 @@ int bhs_stream_copy(int device, long long bytes, int reps, double *ms)
                               int *chosen)                                                     \
     {                                                                                          \
         return apply_tuner<T>(h, net, c, k, chosen);                                           \
+    }                                                                                          \
+    int bhs_##P##_apply_tuner_topk(bhs_handle_t h, bhs_mlp_t net, const bhs_config *c, int k, \
+                                   int topk, int r, long long fb, int *chosen)                 \
+    {                                                                                          \
+        return apply_tuner_topk<T>(h, net, c, k, topk, r, fb, chosen);                         \
     }
 
 BHS_DEFINE_API(s, float)

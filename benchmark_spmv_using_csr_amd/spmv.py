@@ -174,12 +174,19 @@ class bhsparse_spmv:  # noqa: N801  (reference class name)
                                    us.ctypes.data_as(C.c_void_p), C.byref(best)), "autotune")
         return us, best.value
 
-    def apply_tuner(self, net, classes) -> int:
-        """Learned choice on the GPU; installs and returns the class index."""
+    def apply_tuner(self, net, classes, topk=1, reps=7, flush_bytes=1 << 30) -> int:
+        """Learned choice on the GPU; installs and returns the class index.
+        With topk > 1 the network's topk best-predicted classes are timed and
+        the fastest is installed (a learned shortlist for the autotuner)."""
         arr = (Config * len(classes))(*[Config(*c) for c in classes])
         chosen = C.c_int(-1)
-        check(self._fn("apply_tuner")(self._h, net.handle, arr, len(classes), C.byref(chosen)),
-              "apply_tuner")
+        if topk == 1:
+            check(self._fn("apply_tuner")(self._h, net.handle, arr, len(classes), C.byref(chosen)),
+                  "apply_tuner")
+        else:
+            check(self._fn("apply_tuner_topk")(self._h, net.handle, arr, len(classes), int(topk),
+                                               int(reps), int(flush_bytes), C.byref(chosen)),
+                  "apply_tuner_topk")
         return chosen.value
 
     def row_features(self) -> np.ndarray:

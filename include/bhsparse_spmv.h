@@ -199,6 +199,13 @@ int bhs_s_autotune(bhs_handle_t h, const bhs_config *cands, int n_cands, int rep
  * among those that can run is installed (*chosen = its index). */
 int bhs_s_apply_tuner(bhs_handle_t h, bhs_mlp_t net, const bhs_config *classes, int n_classes,
                       int *chosen);
+/* Learned shortlist (no reference counterpart): the topk classes that can run
+ * with the lowest predicted cost are timed as bhs_s_autotune times them
+ * (median of `reps` calls, caches flushed when flush_bytes > 0) and the
+ * fastest is installed (*chosen = its index in classes).  topk = 1 is
+ * bhs_s_apply_tuner; topk = n_classes is the exhaustive autotuner. */
+int bhs_s_apply_tuner_topk(bhs_handle_t h, bhs_mlp_t net, const bhs_config *classes,
+                           int n_classes, int topk, int reps, long long flush_bytes, int *chosen);
 
 /* ---- fp64 build (value_type = double) --------------------------------- */
 int bhs_d_create(bhs_handle_t *h);
@@ -228,6 +235,8 @@ int bhs_d_autotune(bhs_handle_t h, const bhs_config *cands, int n_cands, int rep
                    long long flush_bytes, double *us, int *best);
 int bhs_d_apply_tuner(bhs_handle_t h, bhs_mlp_t net, const bhs_config *classes, int n_classes,
                       int *chosen);
+int bhs_d_apply_tuner_topk(bhs_handle_t h, bhs_mlp_t net, const bhs_config *classes,
+                           int n_classes, int topk, int reps, long long flush_bytes, int *chosen);
 
 /* ---- batched solves (no reference counterpart: the reference runs one
  * matrix and one right-hand side per process, spmv_cuda/main.cu:813-816) ---- */

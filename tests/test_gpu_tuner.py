@@ -84,6 +84,37 @@ def test_apply_tuner_matches_host_forward():
         op.close()
 
 
+def test_learned_shortlist_times_the_best_predicted_classes():
+    """apply_tuner_topk: topk = 1 is the network's pick; topk = 3 installs one of
+    the three best-predicted runnable classes (host forward pass), the result
+    stays within the error bound; bad arguments are refused."""
+    A = matgen.power_law(33, 40000, 40000, 1.9, 8000, 0.1, 0)
+    x = matgen.vector(34, A.n)
+    net = tuner.make_net(tuner.load_net())
+    op = bhsparse_spmv(np.float64)
+    _lib.check(op.init_platform(0), "init_platform")
+    y = np.zeros(A.m)
+    _lib.check(op.prepare_mem(A.m, A.n, A.nnz, A.row_ptr, A.col, A.val, x, y), "prepare_mem")
+    print("[gpu] learned shortlist", flush=True)
+    top1 = op.apply_tuner(net, tuner.CLASSES)
+    assert op.apply_tuner(net, tuner.CLASSES, topk=1) == top1
+    sc = net.forward(tuner.transform(op.row_features()))[0]
+    assert int(np.argmin(sc)) == top1
+    c3 = op.apply_tuner(net, tuner.CLASSES, topk=3, reps=3, flush_bytes=0)
+    # among the three lowest predictions (host and device forward passes may
+    # differ in the last bits, hence the small margin)
+    assert sc[c3] <= np.sort(sc)[2] + 1e-4
+    assert op.get_config().as_tuple() == tuner.CLASSES[c3]
+    _lib.check(op.spmv(1.0), "spmv")
+    yr = oracle.spmv_serial(A.row_ptr, A.col, A.val, x)
+    assert oracle.rel_error(A.row_ptr, A.col, A.val, x, op.y(), yr) <= 1e-13
+    with pytest.raises(_lib.BhsError):
+        op.apply_tuner(net, tuner.CLASSES, topk=0)
+    with pytest.raises(_lib.BhsError):
+        op.apply_tuner(net, tuner.CLASSES, topk=3, reps=0)
+    op.close()
+
+
 def test_autotune_installs_fastest_and_stays_correct():
     A = matgen.uniform_rows(41, 30000, 30000, 0, 40)
     x = matgen.vector(42, A.n)
