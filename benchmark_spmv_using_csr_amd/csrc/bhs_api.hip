@@ -618,11 +618,20 @@ template <typename T>
 void launch_combine(Impl<T> *im)
 {
     if (im->cfg.variant != BHS_VARIANT_SEGSUM_PANELS) return;
-    const int threads = 256;   // one wave per 64 rows
-    const long long blocks = ((long long)im->m + threads - 1) / threads;
-    hipLaunchKernelGGL(bhs::combine_panels_kernel<T>, dim3((unsigned)blocks), dim3(threads), 0,
-                       im->stream, im->d_ystack, im->d_pmask, im->d_pbase, im->m, im->pan_n,
-                       im->d_y);
+    constexpr int kGroups = 2;                 // 64-row groups per wave
+    const int threads = 256;
+    const long long rows_per_block = 64LL * kGroups * (threads / 64);
+    const long long blocks = ((long long)im->m + rows_per_block - 1) / rows_per_block;
+    auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(threads), 0, im->stream,
+                           (const T *)im->d_ystack, (const unsigned char *)im->d_pmask,
+                           (const int *)im->d_pbase, im->m, im->d_y);
+    };
+    switch (im->pan_n) {
+    case 2: go(bhs::combine_panels_kernel<T, 2, kGroups>); break;
+    case 4: go(bhs::combine_panels_kernel<T, 4, kGroups>); break;
+    default: go(bhs::combine_panels_kernel<T, 8, kGroups>); break;
+    }
 }
 
 // TPV rule of main.cu:238-287 / csr_vector_opencl.h:212-230 (by floor(nnz/m)).

@@ -758,31 +758,44 @@ __global__ __launch_bounds__(256) void calibrate_batched_kernel(const int *__res
 
 // y[i] = sum over panels p (in order) of row i's panel partial, +0 where row i
 // has no entry in panel p.  The last step of the column-panel variant; every
-// partial already carries alpha.  One wave per group g of 64 rows: bit p of
-// mask[i] says row i has entries in panel p, and that partial is stacked row
-// base[g*npan+p] + (rows of the group below i with bit p set), a ballot and a
-// popcount.  Reads m bytes of mask instead of an npan x m index map.
-template <typename T>
+// partial already carries alpha.  A wave covers GROUPS groups of 64 rows; for
+// group g, bit p of mask[i] says row i has entries in panel p, and that
+// partial is stacked row base[g*NPAN+p] + (rows of the group below i with bit
+// p set), a ballot and a popcount.  Reads m bytes of mask instead of an
+// NPAN x m index map.  All partials of the wave's rows are loaded before any
+// is added, so each wave has GROUPS*NPAN loads in flight.
+template <typename T, int NPAN, int GROUPS>
 __global__ __launch_bounds__(256) void combine_panels_kernel(const T *__restrict__ ystack,
                                                             const unsigned char *__restrict__ mask,
                                                             const int *__restrict__ base, int m,
-                                                            int npan, T *__restrict__ y)
+                                                            T *__restrict__ y)
 {
     const int lane = lane_id();
-    const long long g = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (g * kWave >= m) return;                       // whole wave past the last row
-    const long long i = g * kWave + lane;
-    const unsigned mk = i < m ? (unsigned)mask[i] : 0u;
-    const int *gb = base + g * npan;
-    T s = T(0);
-    for (int p = 0; p < npan; ++p) {
-        const bool has = (mk >> p) & 1u;
-        const uint64_t bits = ballot(has);             // every lane of the wave is active
-        const int pos = has ? gb[p] + popc_below(bits) : 0;
-        const T v = has ? ystack[pos] : T(0);
-        s = p == 0 ? v : s + v;
+    const long long g0 = (((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * GROUPS;
+    if (g0 * kWave >= m) return;                      // whole wave past the last row
+    T v[GROUPS][NPAN];
+#pragma unroll
+    for (int h = 0; h < GROUPS; ++h) {
+        const long long g = g0 + h;
+        const long long i = g * kWave + lane;
+        const unsigned mk = i < m ? (unsigned)mask[i] : 0u;   // no bits past the last row
+        const int *gb = base + (g * kWave < m ? g : g0) * NPAN;
+#pragma unroll
+        for (int p = 0; p < NPAN; ++p) {
+            const bool has = (mk >> p) & 1u;
+            const uint64_t bits = ballot(has);         // every lane of the wave is active
+            const int pos = has ? gb[p] + popc_below(bits) : 0;
+            v[h][p] = has ? ystack[pos] : T(0);
+        }
     }
-    if (i < m) y[i] = s;
+#pragma unroll
+    for (int h = 0; h < GROUPS; ++h) {
+        const long long i = (g0 + h) * kWave + lane;
+        T s = v[h][0];
+#pragma unroll
+        for (int p = 1; p < NPAN; ++p) s += v[h][p];    // panel order
+        if (i < m) y[i] = s;
+    }
 }
 
 template <typename T>
