@@ -205,7 +205,20 @@ def cpu_baseline(A, x, dtype, y_gpu, budget_s=12.0):
                   f"{b / s1 / 1e9:.3f} GB/s (median of 3 calls, 1 core)",
         "serial_gbps": round(b / s1 / 1e9, 3),
         "gpu_max_rel_err_vs_fp64_oracle": err,
+        "host": {"nproc": os.cpu_count(), "affinity_cores": ncores, "cpu_model": cpu_model()},
     }
+
+
+def cpu_model():
+    """The host CPU's model name (first 'model name' of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
