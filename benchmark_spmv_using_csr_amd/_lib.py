@@ -83,6 +83,7 @@ _PER_TYPE = {
     "get_memory_mode": [_H, C.POINTER(C.c_int)],
     "spmv": [_H, "V"],
     "spmv_async": [_H, "V"],
+    "get_stream": [_H, C.POINTER(C.c_void_p)],
     "get_y": [_H],
     "get_y_to": [_H, "T"],
     "free_mem": [_H],

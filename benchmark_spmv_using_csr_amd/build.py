@@ -86,6 +86,13 @@ def build_example() -> str:
               os.path.join(PKG, "examples", "spmv_small.cpp"), f"-L{PKG}", "-lbhsparse_spmv",
               f"-Wl,-rpath,{PKG}", "-o", tmp])
         os.replace(tmp, out)
+    # hipGraph capture of spmv_async (host code; needs the HIP runtime headers)
+    out = os.path.join(PKG, "examples", "graph_capture")
+    tmp = out + ".tmp"
+    _run([HIPCC, "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}",
+          os.path.join(PKG, "examples", "graph_capture.cpp"), f"-L{PKG}", "-lbhsparse_spmv",
+          f"-Wl,-rpath,{PKG}", "-o", tmp])
+    os.replace(tmp, out)
     return os.path.join(PKG, "examples", "spmv_small")
 
 

@@ -747,6 +747,17 @@ int spmv_async(bhs_handle_t h, T alpha)
 }
 
 template <typename T>
+int get_stream(bhs_handle_t h, void **stream)
+{
+    Impl<T> *im;
+    if (int rc = get_impl(h, &im)) return rc;
+    if (!stream) return set_error(BHSPARSE_INVALID_ARGUMENT, "null stream pointer");
+    if (!im->platform) return set_error(BHSPARSE_NOT_INITIALIZED, "init_platform first");
+    *stream = reinterpret_cast<void *>(im->stream);
+    return BHSPARSE_SUCCESS;
+}
+
+template <typename T>
 int spmv(bhs_handle_t h, T alpha)
 {
     Impl<T> *im;
@@ -1320,6 +1331,7 @@ int bhs_stream_copy(int device, long long bytes, int reps, double *ms)
     int bhs_##P##_get_memory_mode(bhs_handle_t h, int *md) { return get_memory_mode<T>(h, md); } \
     int bhs_##P##_spmv(bhs_handle_t h, T a) { return spmv<T>(h, a); }                         \
     int bhs_##P##_spmv_async(bhs_handle_t h, T a) { return spmv_async<T>(h, a); }             \
+    int bhs_##P##_get_stream(bhs_handle_t h, void **st) { return get_stream<T>(h, st); }      \
     int bhs_##P##_get_y(bhs_handle_t h) { return get_y<T>(h); }                               \
     int bhs_##P##_get_y_to(bhs_handle_t h, T *o) { return get_y_to<T>(h, o); }                \
     int bhs_##P##_free_mem(bhs_handle_t h) { return free_mem<T>(h); }                         \

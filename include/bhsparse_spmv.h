@@ -145,6 +145,12 @@ int bhs_s_get_memory_mode(bhs_handle_t h, int *mode);
 int bhs_s_spmv(bhs_handle_t h, float alpha);
 /* The same work enqueued on the handle's stream without waiting. */
 int bhs_s_spmv_async(bhs_handle_t h, float alpha);
+/* The handle's HIP stream (a hipStream_t), on which bhs_s_spmv_async enqueues.
+ * No reference counterpart.  Once init_data (and set_config, if used) have
+ * run, an spmv_async call enqueues kernel launches only -- no memset, copy or
+ * host step -- so a caller may capture it into a hipGraph and replay it; the
+ * graph reads the handle's current x on every replay. */
+int bhs_s_get_stream(bhs_handle_t h, void **stream);
 /* get_y() (bhsparse_spmv_cuda.h:230-236): copies y into the pointer given to
  * init_data. */
 int bhs_s_get_y(bhs_handle_t h);
@@ -217,6 +223,7 @@ int bhs_d_set_memory_mode(bhs_handle_t h, int mode);
 int bhs_d_get_memory_mode(bhs_handle_t h, int *mode);
 int bhs_d_spmv(bhs_handle_t h, double alpha);
 int bhs_d_spmv_async(bhs_handle_t h, double alpha);
+int bhs_d_get_stream(bhs_handle_t h, void **stream);
 int bhs_d_get_y(bhs_handle_t h);
 int bhs_d_get_y_to(bhs_handle_t h, double *out);
 int bhs_d_free_mem(bhs_handle_t h);

@@ -91,6 +91,10 @@ def test_host_side_argument_checks():
     assert L.bhs_d_get_memory_mode(h, None) == _lib.BHSPARSE_INVALID_ARGUMENT
     assert L.bhs_s_set_memory_mode(h, 1) == _lib.BHSPARSE_UNSUPPORTED_VALUE_TYPE
     assert L.bhs_d_set_memory_mode(h, 0) == 0
+    # the stream exists only after init_platform
+    st = C.c_void_p()
+    assert L.bhs_d_get_stream(h, C.byref(st)) == _lib.BHSPARSE_NOT_INITIALIZED
+    assert L.bhs_d_get_stream(h, None) == _lib.BHSPARSE_INVALID_ARGUMENT
     assert L.bhs_d_free_mem(h) == 0 and L.bhs_d_free_mem(h) == 0     # twice is fine
     assert L.bhs_d_destroy(h) == 0
     assert L.bhs_d_spmv(None, 1.0) == _lib.BHSPARSE_INVALID_ARGUMENT

@@ -28,3 +28,17 @@ def test_reference_call_sequence_in_cpp(name, memory):
     assert r.returncode == 0, r.stdout + r.stderr
     assert any(f"memory: {md}\n" in r.stdout for md in memory), r.stdout
     assert "y = [6, 39, 21, 0, 12, 42] PASS" in r.stdout, r.stdout
+
+
+def test_spmv_async_replays_from_a_captured_hipgraph():
+    """bhs_d_spmv_async captured into a hipGraph (examples/graph_capture.cpp):
+    four replays per variant, with eager calls on the same handle between them,
+    each bitwise equal to the eager result for the x it reads."""
+    exe = os.path.join(EXAMPLES, "graph_capture")
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} is missing: run __graft_entry__.build() first")
+    print("[gpu] C++ graph capture", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("PASS"), r.stdout
+    assert r.stdout.count("match the eager result") == 4, r.stdout
