@@ -42,5 +42,26 @@ int main()
     bhsparse->free_platform();
     bhsparse->free_mem();
     delete bhsparse;
+    if (err || bad) return 1;
+
+    // the OpenCL driver's sequence (spmv_opencl_amd/main.cpp:289-361): init_kernels
+    // after init_platform, get_y checked, sync_device after the timed calls
+    bhsparse_spmv_opencl *ocl = new bhsparse_spmv_opencl();
+    if (ocl->init_kernels("-DUSE_DOUBLE=1") != BHSPARSE_NOT_INITIALIZED) bad++;   // call order
+    err = ocl->init_platform();
+    if (!err) err = ocl->init_kernels("-DUSE_DOUBLE=1");
+    for (int i = 0; i < 6; ++i) y[i] = -1;
+    if (!err) err = ocl->prepare_mem(6, 6, 15, row_ptr, col_idx, val, x, y);
+    if (!err) err = ocl->run_benchmark();
+    if (!err) err = ocl->get_y();
+    for (int i = 0; i < 6 && !err; ++i) bad += y[i] != expect[i];
+    for (int r = 0; r < 10 && !err; ++r) err = ocl->run_benchmark();
+    ocl->sync_device();
+    std::printf("opencl sequence: y = [%g, %g, %g, %g, %g, %g] %s\n", (double)y[0], (double)y[1],
+                (double)y[2], (double)y[3], (double)y[4], (double)y[5],
+                (!err && !bad) ? "PASS" : "FAIL");
+    ocl->free_platform();
+    ocl->free_mem();
+    delete ocl;
     return (err || bad) ? 1 : 0;
 }

@@ -18,6 +18,8 @@
 
 #include "bhsparse_spmv.h"
 
+#include <string>
+
 #ifndef USE_DOUBLE
 #define USE_DOUBLE 1
 #endif
@@ -43,6 +45,7 @@ template <> struct abi<float> {
     static int set_config(bhs_handle_t h, const bhs_config *c) { return bhs_s_set_config(h, c); }
     static int set_memory_mode(bhs_handle_t h, int md) { return bhs_s_set_memory_mode(h, md); }
     static int get_memory_mode(bhs_handle_t h, int *md) { return bhs_s_get_memory_mode(h, md); }
+    static int get_stream(bhs_handle_t h, void **st) { return bhs_s_get_stream(h, st); }
 };
 
 template <> struct abi<double> {
@@ -62,6 +65,7 @@ template <> struct abi<double> {
     static int set_config(bhs_handle_t h, const bhs_config *c) { return bhs_d_set_config(h, c); }
     static int set_memory_mode(bhs_handle_t h, int md) { return bhs_d_set_memory_mode(h, md); }
     static int get_memory_mode(bhs_handle_t h, int *md) { return bhs_d_get_memory_mode(h, md); }
+    static int get_stream(bhs_handle_t h, void **st) { return bhs_d_get_stream(h, st); }
 };
 
 template <typename T>
@@ -117,6 +121,17 @@ public:
     int free_platform() { return abi<T>::free_platform(h_); }
     int free_mem() { return abi<T>::free_mem(h_); }
 
+    // the OpenCL port's two extra methods (spmv_opencl_amd/bhsparse_spmv_opencl.h:11-19).
+    // The kernels are compiled ahead of time for gfx950, so init_kernels only
+    // checks the call order; its compile flags are not used.
+    int init_kernels(const std::string & /*compile_flags*/)
+    {
+        void *st = nullptr;
+        return abi<T>::get_stream(h_, &st);   // BHSPARSE_NOT_INITIALIZED before init_platform
+    }
+    // run_benchmark already waits for its kernels; this waits for the device.
+    void sync_device() { bhs_device_synchronize(); }
+
     bhs_handle_t handle() const { return h_; }
 
 private:
@@ -136,5 +151,7 @@ typedef bhsparse::spmv_operator<value_type> bhsparse_spmv;
 // The reference driver's class name (spmv_cuda/main.cu:772, :490), so that
 // `new bhsparse_spmv_cuda()` compiles unchanged against this header.
 typedef bhsparse_spmv bhsparse_spmv_cuda;
+// and the OpenCL driver's (spmv_opencl_amd/main.cpp:289)
+typedef bhsparse_spmv bhsparse_spmv_opencl;
 
 #endif // BHSPARSE_SPMV_HPP

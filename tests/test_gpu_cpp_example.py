@@ -28,6 +28,8 @@ def test_reference_call_sequence_in_cpp(name, memory):
     assert r.returncode == 0, r.stdout + r.stderr
     assert any(f"memory: {md}\n" in r.stdout for md in memory), r.stdout
     assert "y = [6, 39, 21, 0, 12, 42] PASS" in r.stdout, r.stdout
+    # and the OpenCL driver's sequence (init_kernels, get_y checked, sync_device)
+    assert "opencl sequence: y = [6, 39, 21, 0, 12, 42] PASS" in r.stdout, r.stdout
 
 
 def test_spmv_async_replays_from_a_captured_hipgraph():
