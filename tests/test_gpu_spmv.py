@@ -136,6 +136,32 @@ def test_row_chunk_choice(S):
         assert np.array_equal(y, km.segsum(c.row_ptr, c.col, c.val, c.x, S=S, TPW=3))
 
 
+@pytest.mark.parametrize("S", [4, 8])
+def test_long_tiles_under_the_256_row_chunk(S):
+    """Tiles longer than the 256-row chunk (walked in the long-tile loop, empty
+    runs skipped by galloping) on a matrix that takes the 256-row chunk: runs
+    of 300-5,000 empty rows and of length-1 rows among ~190-row tiles."""
+    rng = np.random.default_rng(100 + S)
+    Lm = 64 * S / 190
+    parts = []
+    for blk in range(12):
+        lens = (np.floor(Lm) + (rng.random(1500) < Lm - np.floor(Lm))).astype(np.int64)
+        parts.append(lens)
+        parts.append(np.zeros(int(rng.choice([300, 700, 5000])), np.int64))
+        parts.append(np.ones(int(rng.choice([257, 400])), np.int64))
+    lens = np.concatenate(parts)
+    c = cases.from_lengths("long tiles, 256-row chunk", lens, 3000, seed=S)
+    say(f"long tiles under the 256-row chunk S={S} m={c.m}")
+    y, op = gpu_spmv(c, np.float64, (S, 4, 3))
+    assert op.stats()["chunk_rows"] == expected_chunk_rows(c, S) == 256
+    span = np.diff(np.asarray(oracle.partition(c.row_ptr, c.nnz, 64 * S), np.int64))
+    assert (span >= 256).any()
+    assert np.array_equal(y, km.segsum(c.row_ptr, c.col, c.val, c.x, S=S, TPW=3))
+    v, x = c.val, c.x
+    yr = oracle.spmv_serial(c.row_ptr, c.col, v, x)
+    assert oracle.rel_error(c.row_ptr, c.col, v, x, y, yr) <= 1e-13
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_repeatable_and_independent_of_previous_y(dtype):
     A = matgen.power_law(13, 50000, 50000, 2.0, 20000, 0.1, 0)
