@@ -83,6 +83,13 @@ struct Impl {
     int *d_prow_ptr = nullptr, *d_pcol = nullptr, *d_pbase = nullptr;
     unsigned char *d_pmask = nullptr;
     T *d_pval = nullptr, *d_ystack = nullptr;
+    // host copies for the chain records: stacked row_ptr, the row of A each
+    // stacked row belongs to, and the first stacked row of every panel
+    std::vector<int> h_prow_ptr, h_prow_of, h_pan_start;
+    // chain records of the combine (one per combine wave, then the overflow),
+    // rebuilt with every analysis of the panel layout
+    bhs::ChainRec *d_chains = nullptr;
+    long long chain_cap = 0;
 
     // analysis for the current tile shape and layout (an_layout = panels, 0 = A)
     int an_P = 0, an_TPW = 0, an_layout = -1;
@@ -213,6 +220,11 @@ void release_data(Impl<T> *im)
     free_ptr(im->d_ystack);
     free_ptr(im->d_pmask);
     free_ptr(im->d_pbase);
+    free_ptr(im->d_chains);
+    im->chain_cap = 0;
+    im->h_prow_ptr = {};
+    im->h_prow_of = {};
+    im->h_pan_start = {};
     im->pan_n = im->pan_rows = 0;
     im->floor_cap = 0;
     im->tile_cap = im->sync_cap = 0;
@@ -303,6 +315,9 @@ int build_panels(Impl<T> *im, int npan)
     free_ptr(im->d_ystack);
     free_ptr(im->d_pmask);
     free_ptr(im->d_pbase);
+    im->h_prow_ptr = {};
+    im->h_prow_of = {};
+    im->h_pan_start = {};
     im->pan_n = im->pan_rows = 0;
     const int m = im->m, n = im->n, nnz = im->nnz;
     if ((long long)npan * m > 0x7fffffffLL)
@@ -323,17 +338,20 @@ int build_panels(Impl<T> *im, int npan)
     std::vector<int> cnt(PM, 0);   // entries of row i in panel p, at p*m+i
     for (int i = 0; i < m; ++i)
         for (int j = rp[i]; j < rp[i + 1]; ++j) ++cnt[(size_t)(col[j] / pw) * m + i];
-    std::vector<int> idx(PM), prp(1, 0);
+    std::vector<int> idx(PM), prp(1, 0), prow_of, pan_start((size_t)npan + 1, 0);
     prp.reserve(PM + 1);
     for (size_t q = 0; q < PM; ++q) {
+        if (q % (size_t)m == 0) pan_start[q / (size_t)m] = (int)prp.size() - 1;
         if (cnt[q] > 0) {
             idx[q] = (int)prp.size() - 1;
             prp.push_back(prp.back() + cnt[q]);
+            prow_of.push_back((int)(q % (size_t)m));
         } else {
             idx[q] = -1;
         }
     }
     const int M = (int)prp.size() - 1;
+    pan_start[(size_t)npan] = M;
     // combine metadata: per-row panel bits, per-(64-row group, panel) base
     const long long G = ((long long)m + 63) / 64;
     std::vector<unsigned char> mask((size_t)m, 0);
@@ -383,8 +401,69 @@ int build_panels(Impl<T> *im, int npan)
                                hipMemcpyHostToDevice, im->stream));
     }
     BHS_HIP(hipStreamSynchronize(im->stream));
+    im->h_prow_ptr = std::move(prp);
+    im->h_prow_of = std::move(prow_of);
+    im->h_pan_start = std::move(pan_start);
     im->pan_n = npan;
     im->pan_rows = M;
+    return BHSPARSE_SUCCESS;
+}
+
+// Chain records of the panel combine for waves of PW stacked nonzeros.  Wave
+// w > 0 leaves a partial for stacked row R when R = upper_bound(row_ptr,
+// w*PW) - 1 began before w*PW (the segmented sum's `cont` test on its first
+// tile; the stacked matrix has no empty rows, so R is the row holding nonzero
+// w*PW); consecutive such waves with the same R form one chain.
+template <typename T>
+int build_chains(Impl<T> *im, long long PW, long long NW)
+{
+    const std::vector<int> &rp = im->h_prow_ptr;
+    const long long groups = 64LL * bhs::kCombineGroups;
+    const long long n_cw = ((long long)im->m + groups - 1) / groups;
+    std::vector<bhs::ChainRec> rec((size_t)std::max<long long>(n_cw, 1), bhs::ChainRec{0, 0, 0, -1});
+    auto add = [&](int R, int first, int count) {
+        const int i = im->h_prow_of[(size_t)R];
+        const int pan = (int)(std::upper_bound(im->h_pan_start.begin(), im->h_pan_start.end(), R) -
+                              im->h_pan_start.begin()) - 1;
+        const long long cw = i / groups;
+        bhs::ChainRec c{first, count, (int)(i % groups) << 4 | pan, -1};
+        if (rec[(size_t)cw].count == 0) {
+            rec[(size_t)cw] = c;
+        } else {                              // second in the combine wave's list
+            c.next = rec[(size_t)cw].next;
+            rec.push_back(c);
+            rec[(size_t)cw].next = (int)rec.size() - 1;
+        }
+    };
+    int R_open = -1, first = 0, count = 0;
+    for (long long w = 1; w < NW; ++w) {
+        const long long s = w * PW;
+        const int R = (int)(std::upper_bound(rp.begin(), rp.end(), (int)s) - rp.begin()) - 1;
+        const bool sync = rp[(size_t)R] < s;
+        if (sync && R == R_open && first + count == w) {
+            ++count;
+            continue;
+        }
+        if (count > 0) add(R_open, first, count);
+        R_open = -1;
+        count = 0;
+        if (sync) {
+            R_open = R;
+            first = (int)w;
+            count = 1;
+        }
+    }
+    if (count > 0) add(R_open, first, count);
+    if ((long long)rec.size() > 0x7fffffffLL)
+        return set_error(BHSPARSE_INVALID_ARGUMENT, "too many chain records");
+    if ((long long)rec.size() > im->chain_cap) {
+        free_ptr(im->d_chains);
+        BHS_HIP(hipMalloc(&im->d_chains, sizeof(bhs::ChainRec) * rec.size()));
+        im->chain_cap = (long long)rec.size();
+    }
+    BHS_HIP(hipMemcpyAsync(im->d_chains, rec.data(), sizeof(bhs::ChainRec) * rec.size(),
+                           hipMemcpyHostToDevice, im->stream));
+    BHS_HIP(hipStreamSynchronize(im->stream));
     return BHSPARSE_SUCCESS;
 }
 
@@ -433,6 +512,8 @@ int ensure_analysis(Impl<T> *im)
         BHS_HIP(hipMemcpyAsync(&mid_tiles, im->d_counters, sizeof(mid_tiles),
                                hipMemcpyDeviceToHost, im->stream));
     }
+    if (layout > 0 && im->m > 0)
+        if (int rc = build_chains(im, (long long)P * TPW, NW)) return rc;
     int first = 0;
     BHS_HIP(hipMemcpyAsync(&first, im->d_tile_row, sizeof(int), hipMemcpyDeviceToHost, im->stream));
     BHS_HIP(hipStreamSynchronize(im->stream));
@@ -604,10 +685,12 @@ void launch_segsum(Impl<T> *im, T alpha, unsigned long long *stats, hipEvent_t e
     else launch_segsum_cp<T, STATS, 2>(im, alpha, stats, e0, e1);
 }
 
+// The column-panel layout is calibrated inside its combine.
 template <typename T>
 void launch_calibrate(Impl<T> *im)
 {
     if (im->NW <= 1) return;   // wave 0 never has a boundary partial
+    if (im->cfg.variant == BHS_VARIANT_SEGSUM_PANELS) return;
     const int threads = 256;
     hipLaunchKernelGGL(bhs::calibrate_kernel<T>, dim3((im->NW + threads - 1) / threads),
                        dim3(threads), 0, im->stream, im->d_sync_row, im->d_sync_val, im->NW,
@@ -618,14 +701,15 @@ template <typename T>
 void launch_combine(Impl<T> *im)
 {
     if (im->cfg.variant != BHS_VARIANT_SEGSUM_PANELS) return;
-    constexpr int kGroups = 2;                 // 64-row groups per wave
+    constexpr int kGroups = bhs::kCombineGroups;   // 64-row groups per wave
     const int threads = 256;
     const long long rows_per_block = 64LL * kGroups * (threads / 64);
     const long long blocks = ((long long)im->m + rows_per_block - 1) / rows_per_block;
     auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(threads), 0, im->stream,
                            (const T *)im->d_ystack, (const unsigned char *)im->d_pmask,
-                           (const int *)im->d_pbase, im->m, im->d_y);
+                           (const int *)im->d_pbase, im->m, im->d_y,
+                           (const bhs::ChainRec *)im->d_chains, (const T *)im->d_sync_val);
     };
     switch (im->pan_n) {
     case 2: go(bhs::combine_panels_kernel<T, 2, kGroups>); break;

@@ -764,29 +764,77 @@ __global__ __launch_bounds__(256) void calibrate_batched_kernel(const int *__res
 // p set), a ballot and a popcount.  Reads m bytes of mask instead of an
 // NPAN x m index map.  All partials of the wave's rows are loaded before any
 // is added, so each wave has GROUPS*NPAN loads in flight.
+//
+// The calibration of the stacked matrix is done here too (no calibrate_kernel
+// launch on this layout).  A stacked row whose nonzeros cross wave boundaries
+// has a chain of consecutive waves that each left a partial in sync_val; the
+// chains depend only on the stacked matrix and the tile shape, so they are
+// listed once per analysis as ChainRec records: chains[cw] is the first chain
+// landing in combine wave cw (count 0: none), `next` links the others.  The
+// chain's total is summed 64 entries at a time (wave_sum of each chunk, the
+// chunk sums in order) and added to that partial before the panel sum:
+// y_stack[R] + total, the value calibrate_kernel would have stored.
+struct ChainRec {
+    int first;   // first wave of the chain (its sync_val index)
+    int count;   // waves in the chain (0: no record)
+    int where;   // (row - first row of the combine wave) << 4 | panel
+    int next;    // index of the next record of the same combine wave, -1: none
+};
+constexpr int kCombineGroups = 2;   // 64-row groups per combine wave
+
 template <typename T, int NPAN, int GROUPS>
 __global__ __launch_bounds__(256) void combine_panels_kernel(const T *__restrict__ ystack,
                                                             const unsigned char *__restrict__ mask,
                                                             const int *__restrict__ base, int m,
-                                                            T *__restrict__ y)
+                                                            T *__restrict__ y,
+                                                            const ChainRec *__restrict__ chains,
+                                                            const T *__restrict__ sync_val)
 {
     const int lane = lane_id();
-    const long long g0 = (((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6) * GROUPS;
+    const int cw = __builtin_amdgcn_readfirstlane(
+        (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    const long long g0 = (long long)cw * GROUPS;
     if (g0 * kWave >= m) return;                      // whole wave past the last row
-    T v[GROUPS][NPAN];
+    ChainRec c = chains[cw];                          // wave-uniform
+    // masks and bases of every group first, so that no wait for a mask also
+    // waits for partials already in flight
+    unsigned mk[GROUPS];
+    int gb[GROUPS][NPAN];
 #pragma unroll
     for (int h = 0; h < GROUPS; ++h) {
         const long long g = g0 + h;
         const long long i = g * kWave + lane;
-        const unsigned mk = i < m ? (unsigned)mask[i] : 0u;   // no bits past the last row
-        const int *gb = base + (g * kWave < m ? g : g0) * NPAN;
+        mk[h] = i < m ? (unsigned)mask[i] : 0u;          // no bits past the last row
+        const int *b = base + (g * kWave < m ? g : g0) * NPAN;
+#pragma unroll
+        for (int p = 0; p < NPAN; ++p) gb[h][p] = b[p];
+    }
+    T v[GROUPS][NPAN];
+#pragma unroll
+    for (int h = 0; h < GROUPS; ++h) {
 #pragma unroll
         for (int p = 0; p < NPAN; ++p) {
-            const bool has = (mk >> p) & 1u;
+            const bool has = (mk[h] >> p) & 1u;
             const uint64_t bits = ballot(has);         // every lane of the wave is active
-            const int pos = has ? gb[p] + popc_below(bits) : 0;
+            const int pos = has ? gb[h][p] + popc_below(bits) : 0;
             v[h][p] = has ? ystack[pos] : T(0);
         }
+    }
+    while (c.count > 0) {                             // wave-uniform loop
+        T tot = T(0);
+        for (int j = 0; j < c.count; j += kWave) {
+            const T s = wave_sum(j + lane < c.count ? sync_val[(long long)c.first + j + lane]
+                                                    : T(0));
+            tot = j == 0 ? s : tot + s;
+        }
+        const int slot = c.where >> 4, pan = c.where & 15;
+#pragma unroll
+        for (int h = 0; h < GROUPS; ++h)
+#pragma unroll
+            for (int p = 0; p < NPAN; ++p)
+                if (pan == p && slot == h * kWave + lane) v[h][p] = v[h][p] + tot;
+        if (c.next < 0) break;
+        c = chains[c.next];
     }
 #pragma unroll
     for (int h = 0; h < GROUPS; ++h) {

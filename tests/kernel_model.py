@@ -92,8 +92,36 @@ def _calibrate(sync_row, sync_val, NW, y, T):
                 y[r[i]] = T(y[r[i]] + tot)
 
 
-def segsum(row_ptr, col, val, x, S=8, TPW=3, alpha=1.0, dtype=np.float64, counters=None):
-    """y = alpha*A*x exactly as segsum_kernel + calibrate_kernel compute it."""
+def calibrate_chains(sync_row, sync_val, y):
+    """The column-panel combine's calibration: every run of consecutive sync
+    entries with the same row R (a chain) is summed 64 entries at a time --
+    _wave_sum of each zero-padded chunk, the chunk sums added in order -- and
+    y[R] becomes y[R] + total."""
+    T = y.dtype.type
+    NW = len(sync_row)
+    w = 0
+    while w < NW:
+        r = int(sync_row[w])
+        if r < 0:
+            w += 1
+            continue
+        e = w
+        while e < NW and sync_row[e] == r:
+            e += 1
+        tot = None
+        for j in range(w, e, W):
+            chunk = np.zeros(W, y.dtype)
+            chunk[:min(W, e - j)] = sync_val[j:min(j + W, e)]
+            s = T(_wave_sum(chunk))
+            tot = s if tot is None else T(tot + s)
+        y[r] = T(y[r] + tot)
+        w = e
+
+
+def segsum(row_ptr, col, val, x, S=8, TPW=3, alpha=1.0, dtype=np.float64, counters=None,
+           calibrate=True):
+    """y = alpha*A*x exactly as segsum_kernel + calibrate_kernel compute it.
+    With calibrate=False: (y before calibration, sync_row, sync_val)."""
     T = np.dtype(dtype).type
     row_ptr = np.asarray(row_ptr, np.int64)
     m = len(row_ptr) - 1
@@ -106,11 +134,12 @@ def segsum(row_ptr, col, val, x, S=8, TPW=3, alpha=1.0, dtype=np.float64, counte
     st = {"n_fast": 0, "n_dirty": 0, "n_sync": 0}
     if counters is not None:
         counters.update(st)
+    none = (np.zeros(0, np.int64), np.zeros(0, dtype))
     if m == 0:
-        return y
+        return y if calibrate else (y,) + none
     if NP == 0:
         y[:] = 0
-        return y
+        return y if calibrate else (y,) + none
     tile_row = partition(row_ptr, nnz, P)
     lead_empty = int(tile_row[0])
     pad = NP * P
@@ -233,7 +262,29 @@ def segsum(row_ptr, col, val, x, S=8, TPW=3, alpha=1.0, dtype=np.float64, counte
             sync_row[w] = -1
         if did_sync:
             st["n_sync"] += 1
-    _calibrate(sync_row, sync_val, NW, y, T)
     if counters is not None:
         counters.update(st)
+    if not calibrate:
+        return y, sync_row, sync_val
+    _calibrate(sync_row, sync_val, NW, y, T)
     return y
+
+
+def segsum_panels(st_row_ptr, st_col, st_val, idx, x, S=8, TPW=3, dtype=np.float64):
+    """The column-panel variant: segsum_kernel over the stacked matrix (no
+    calibrate_kernel), then combine_panels_kernel -- its chain calibration and
+    the sum over panels in panel order.  idx (npan x m) holds the stacked row
+    of every (panel, row) pair, -1 where the row has no entry in the panel."""
+    T = np.dtype(dtype).type
+    npan, m = idx.shape
+    if len(st_row_ptr) > 1:
+        ys, sr, sv = segsum(st_row_ptr, st_col, np.asarray(st_val, dtype), np.asarray(x, dtype),
+                            S=S, TPW=TPW, dtype=dtype, calibrate=False)
+        calibrate_chains(sr, sv, ys)
+    else:
+        ys = np.zeros(1, dtype)
+    part = lambda p: np.where(idx[p] >= 0, ys[np.maximum(idx[p], 0)], T(0)).astype(dtype)
+    out = part(0)
+    for p in range(1, npan):
+        out = (out + part(p)).astype(dtype)
+    return out

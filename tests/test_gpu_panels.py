@@ -1,9 +1,10 @@
 """The column-panel variant (BHS_VARIANT_SEGSUM_PANELS) through the C ABI:
 partition bit-exact on the stacked matrix, y bit-exact against the lane-exact
-model run on the stacked matrix and summed over panels in order, y within the
-error bound of the FP64 oracle (exact on integer data), consistent when the
-handle switches between layouts, and no slower than a few times the plain
-layout on banded matrices (every stacked row is nonempty)."""
+model run on the stacked matrix (its wave-boundary partials added in the
+combine, chunk by chunk) and summed over panels in order, y within the error
+bound of the FP64 oracle (exact on integer data), consistent when the handle
+switches between layouts, and no slower than a few times the plain layout on
+banded matrices (every stacked row is nonempty)."""
 import numpy as np
 import pytest
 
@@ -41,14 +42,7 @@ def run(c, dtype, cfg, alpha=1.0):
 def model(c, dtype, cfg):
     S, _, tpw, npan = cfg
     st, idx = cases.stack_panels(c, npan)
-    ys = km.segsum(st.row_ptr, st.col, st.val.astype(dtype), c.x.astype(dtype), S=S, TPW=tpw,
-                   dtype=dtype) if st.m > 0 else np.zeros(0, dtype)
-    part = lambda p: np.where(idx[p] >= 0, ys[np.maximum(idx[p], 0)] if st.m else 0,
-                              0).astype(dtype)
-    out = part(0)
-    for p in range(1, npan):
-        out = (out + part(p)).astype(dtype)
-    return out
+    return km.segsum_panels(st.row_ptr, st.col, st.val, idx, c.x, S=S, TPW=tpw, dtype=dtype)
 
 
 @pytest.mark.parametrize("cfg", CONFIGS)
@@ -68,6 +62,47 @@ def test_adversarial_against_model_and_oracle(cfg, dtype):
             if integer:
                 assert np.array_equal(y.astype(np.float64), yr), c.name
             assert oracle.rel_error(c.row_ptr, c.col, v, x, y, yr) <= TOL[dtype], c.name
+
+
+def long_chain_case(seed, wave_nnz, npan):
+    """Short random rows around two long ones: in every panel the first long
+    row spans about 150 waves (three 64-entry chunks of wave partials, the last
+    one partial) and the second exactly 64, next to rows spanning 2-3 waves."""
+    rng = np.random.default_rng(seed)
+    n = 40000
+    lens = list(rng.integers(0, 12, 3000))
+    lens[100] = 150 * wave_nnz * npan + 37
+    lens[2000] = 64 * wave_nnz * npan
+    lens[2500] = 2 * wave_nnz * npan + 5
+    lens[2900] = 3 * wave_nnz * npan
+    rp = np.zeros(len(lens) + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    # long rows: every column in order, repeated in blocks so that each panel
+    # (a contiguous range of n / npan columns) gets the same share
+    col = np.empty(int(rp[-1]), np.int32)
+    for i, L in enumerate(lens):
+        if L > n:
+            col[rp[i]:rp[i + 1]] = np.sort(np.arange(L, dtype=np.int64) * n // L)
+        else:
+            col[rp[i]:rp[i + 1]] = np.sort(rng.choice(n, L, replace=False))
+    val = rng.integers(-8, 9, int(rp[-1])).astype(np.float64)
+    x = rng.integers(-8, 9, n).astype(np.float64)
+    return cases.Case("long chains", len(lens), n, rp.astype(np.int32), col, val, x)
+
+
+@pytest.mark.parametrize("cfg", [(4, 1, 1, 2), (8, 2, 1, 4)])
+def test_wave_chains_longer_than_a_chunk(cfg):
+    """Rows whose partials span 64 and more waves: the combine sums a chain in
+    chunks of 64 wave partials; bit-exact against the model and, on integer
+    data, exactly the oracle's y."""
+    S, _, tpw, npan = cfg
+    c = long_chain_case(S, 64 * S * tpw, npan)
+    say(f"panels long chains cfg={cfg} nnz={c.nnz}")
+    y, op = run(c, np.float64, cfg)
+    op.close()
+    assert np.array_equal(y, model(c, np.float64, cfg))
+    yr = oracle.spmv_serial(c.row_ptr, c.col, c.val, c.x)
+    assert np.array_equal(y, yr)
 
 
 @pytest.mark.parametrize("npan", [2, 4, 8])
