@@ -149,7 +149,9 @@ int bhs_s_spmv_async(bhs_handle_t h, float alpha);
  * No reference counterpart.  Once init_data (and set_config, if used) have
  * run, an spmv_async call enqueues kernel launches only -- no memset, copy or
  * host step -- so a caller may capture it into a hipGraph and replay it; the
- * graph reads the handle's current x on every replay. */
+ * graph reads the handle's current x on every replay.  The graph holds the
+ * buffers of the tile shape and layout it was captured with: capture again
+ * after set_config, init_data or free_mem. */
 int bhs_s_get_stream(bhs_handle_t h, void **stream);
 /* get_y() (bhsparse_spmv_cuda.h:230-236): copies y into the pointer given to
  * init_data. */
