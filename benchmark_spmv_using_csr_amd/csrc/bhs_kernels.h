@@ -229,6 +229,33 @@ struct MatView {
     int m, nnz, NP, NW, lead_empty, k, kstride, TPW, key_base;
 };
 
+// Several right-hand sides (KMAX > 1): the gathers of RHS rr+1 are issued
+// before RHS rr is reduced, so their latency overlaps that work instead of
+// following it.  Each RHS's arithmetic is unchanged.  With KMAX == 1 both
+// helpers compile to nothing.
+template <typename T, int S, int KMAX>
+__device__ __forceinline__ void gather_next_rhs(const T *__restrict__ x, const Tile<T, S> &t,
+                                                int ks, int rr, int k, T (&xn)[S])
+{
+    if constexpr (KMAX > 1) {
+        if (rr + 1 < k) {                        // wave-uniform
+#pragma unroll
+            for (int kk = 0; kk < S; ++kk) xn[kk] = x[(long long)t.c[kk] * ks + rr + 1];
+        }
+    }
+}
+
+template <typename T, int S, int KMAX>
+__device__ __forceinline__ void next_rhs(int rr, int k, const T (&xn)[S], T (&xv)[S])
+{
+    if constexpr (KMAX > 1) {
+        if (rr + 1 < k) {
+#pragma unroll
+            for (int kk = 0; kk < S; ++kk) xv[kk] = xn[kk];
+        }
+    }
+}
+
 template <typename T, int S, int KMAX, bool STRIDED, bool STATS, int CP>
 __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, const int lane,
                                             unsigned *s_head, T *s_buf, T *s_carry, const T alpha,
@@ -310,10 +337,8 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
 
         if (!general) {                          // fast track
             for (int rr = 0; rr < k; ++rr) {
-                if (rr > 0) {
-#pragma unroll
-                    for (int kk = 0; kk < S; ++kk) xv[kk] = x[(long long)cur.c[kk] * ks + rr];
-                }
+                T xn[S];
+                gather_next_rhs<T, S, KMAX>(x, cur, ks, rr, k, xn);
                 T prod[S];
 #pragma unroll
                 for (int kk = 0; kk < S; ++kk) {
@@ -325,6 +350,7 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                 for (int kk = 1; kk < S; ++kk) s += prod[kk];
                 s = wave_sum(s);
                 carry_set(rr, cont ? carry_get(rr) + s : s);
+                next_rhs<T, S, KMAX>(rr, k, xn, xv);
             }
             open = true;
             if (STATS) ++n_fast;
@@ -421,10 +447,8 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                 if (STATS && dirty) ++n_dirty;
 
                 for (int rr = 0; rr < k; ++rr) {
-                    if (rr > 0) {
-    #pragma unroll
-                        for (int kk = 0; kk < S; ++kk) xv[kk] = x[(long long)cur.c[kk] * ks + rr];
-                    }
+                    T xn[S];
+                    gather_next_rhs<T, S, KMAX>(x, cur, ks, rr, k, xn);
                     T prod[S];
     #pragma unroll
                     for (int kk = 0; kk < S; ++kk) {
@@ -512,6 +536,7 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                         else if (lane == 0) mv.sync_val[(long long)w * KMAX + rr] = alpha * s_buf[0];
                     }
                     carry_set(rr, s_buf[K - 1]);
+                    next_rhs<T, S, KMAX>(rr, k, xn, xv);
                 }
                 if (syncing) {
                     sync_key = mv.key_base + r0 * ks;
