@@ -229,32 +229,77 @@ struct MatView {
     int m, nnz, NP, NW, lead_empty, k, kstride, TPW, key_base;
 };
 
-// Several right-hand sides (KMAX > 1): the gathers of RHS rr+1 are issued
-// before RHS rr is reduced, so their latency overlaps that work instead of
-// following it.  Each RHS's arithmetic is unchanged.  With KMAX == 1 both
-// helpers compile to nothing.
+// x values of the tile's items for the right-hand side being reduced (v).
+// One right-hand side: a gather per item.  Several (KMAX > 1): the gathers
+// of the next right-hand side are issued before the current one is reduced,
+// so their latency overlaps that work.  When the stride between a column's
+// right-hand sides and their number are even, RHS rr and rr+1 (rr even) of
+// an item come from one aligned 2-wide load, half the gather requests; that
+// pair is issued while RHS rr-1 is reduced (RHS 0 and 1 with the tile's other
+// loads).  Each RHS's arithmetic is unchanged.
 template <typename T, int S, int KMAX>
-__device__ __forceinline__ void gather_next_rhs(const T *__restrict__ x, const Tile<T, S> &t,
-                                                int ks, int rr, int k, T (&xn)[S])
-{
-    if constexpr (KMAX > 1) {
-        if (rr + 1 < k) {                        // wave-uniform
-#pragma unroll
-            for (int kk = 0; kk < S; ++kk) xn[kk] = x[(long long)t.c[kk] * ks + rr + 1];
-        }
-    }
-}
+struct RhsX {
+    T v[S];                                     // RHS rr
+    T n0[S], n1[S];                             // KMAX > 1: RHS rr+1, rr+2
 
-template <typename T, int S, int KMAX>
-__device__ __forceinline__ void next_rhs(int rr, int k, const T (&xn)[S], T (&xv)[S])
-{
-    if constexpr (KMAX > 1) {
-        if (rr + 1 < k) {
+    __device__ __forceinline__ void load1(const T *__restrict__ x, const Tile<T, S> &t, int ks,
+                                          int rr, T (&d)[S])
+    {
 #pragma unroll
-            for (int kk = 0; kk < S; ++kk) xv[kk] = xn[kk];
+        for (int kk = 0; kk < S; ++kk) d[kk] = x[(long long)t.c[kk] * ks + rr];
+    }
+    __device__ __forceinline__ void load2(const T *__restrict__ x, const Tile<T, S> &t, int ks,
+                                          int rr, T (&d0)[S], T (&d1)[S])
+    {
+        using P2 = T __attribute__((ext_vector_type(2)));
+#pragma unroll
+        for (int kk = 0; kk < S; ++kk) {
+            const P2 q = *reinterpret_cast<const P2 *>(x + (long long)t.c[kk] * ks + rr);
+            // d1 first: load1 and load2 then both end with a store to d0, so
+            // the compiler's merge of the two branches never selects between
+            // two arrays (which would put them in scratch memory)
+            d1[kk] = q.y;
+            d0[kk] = q.x;
         }
     }
-}
+    // RHS 0 (and RHS 1, into n0) of a tile
+    __device__ __forceinline__ void first(const T *__restrict__ x, const Tile<T, S> &t, int ks,
+                                          int k, bool pairs)
+    {
+        if constexpr (KMAX > 1) {
+            if (pairs && k > 1) {
+                load2(x, t, ks, 0, v, n0);
+                return;
+            }
+        }
+        load1(x, t, ks, 0, v);
+    }
+    // At the top of RHS rr: issue the loads of RHS rr+1 unless they are in
+    // n0 already (the second of a pair).  n0 receives RHS rr+1, n1 RHS rr+2.
+    // All branches are wave-uniform.
+    __device__ __forceinline__ void issue(const T *__restrict__ x, const Tile<T, S> &t, int ks,
+                                          int rr, int k, bool pairs)
+    {
+        if constexpr (KMAX > 1) {
+            if (rr + 1 >= k) return;
+            if (!pairs) load1(x, t, ks, rr + 1, n0);
+            else if (rr & 1) load2(x, t, ks, rr + 1, n0, n1);   // (rr+1, rr+2); k is even
+        }
+    }
+    // At the end of RHS rr: v becomes RHS rr+1, and after a pair n0 its second.
+    __device__ __forceinline__ void advance(int rr, int k, bool pairs)
+    {
+        if constexpr (KMAX > 1) {
+            if (rr + 1 >= k) return;
+#pragma unroll
+            for (int kk = 0; kk < S; ++kk) v[kk] = n0[kk];
+            if (pairs && (rr & 1)) {
+#pragma unroll
+                for (int kk = 0; kk < S; ++kk) n0[kk] = n1[kk];
+            }
+        }
+    }
+};
 
 template <typename T, int S, int KMAX, bool STRIDED, bool STATS, int CP>
 __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, const int lane,
@@ -265,6 +310,9 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
     constexpr int G = Tile<T, S>::G;
     const int k = (KMAX == 1) ? 1 : mv.k;
     const int ks = STRIDED ? mv.kstride : 1;
+    // 2-wide x loads: whole pairs of right-hand sides (k even) at an even
+    // stride, from x offsets that keep them aligned
+    const bool pairs = KMAX > 1 && (ks & 1) == 0 && (k & 1) == 0;
     const int *__restrict__ row_ptr = mv.row_ptr;
     const int *__restrict__ tile_row = mv.tile_row;
     const T *__restrict__ x = mv.x;
@@ -322,9 +370,8 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
         const int r0 = r1;
         r1 = tile_row[p + 1];
         const bool general = r0 != r1;
-        T xv[S];
-#pragma unroll
-        for (int kk = 0; kk < S; ++kk) xv[kk] = x[(long long)cur.c[kk] * ks];
+        RhsX<T, S, KMAX> xr;
+        xr.first(x, cur, ks, k, pairs);
         const RowChunk<CP> cc = rc;
         rc = load_chunk<CP>(row_ptr, r1, lane, mv.m);
         __builtin_amdgcn_sched_barrier(0);
@@ -337,12 +384,11 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
 
         if (!general) {                          // fast track
             for (int rr = 0; rr < k; ++rr) {
-                T xn[S];
-                gather_next_rhs<T, S, KMAX>(x, cur, ks, rr, k, xn);
+                xr.issue(x, cur, ks, rr, k, pairs);
                 T prod[S];
 #pragma unroll
                 for (int kk = 0; kk < S; ++kk) {
-                    const T pk = cur.v[kk] * xv[kk];
+                    const T pk = cur.v[kk] * xr.v[kk];
                     prod[kk] = tail ? keep_or_zero(pk, tile_item<T, S>(B, lane, kk) < mv.nnz) : pk;
                 }
                 T s = prod[0];
@@ -350,7 +396,7 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                 for (int kk = 1; kk < S; ++kk) s += prod[kk];
                 s = wave_sum(s);
                 carry_set(rr, cont ? carry_get(rr) + s : s);
-                next_rhs<T, S, KMAX>(rr, k, xn, xv);
+                xr.advance(rr, k, pairs);
             }
             open = true;
             if (STATS) ++n_fast;
@@ -447,12 +493,11 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                 if (STATS && dirty) ++n_dirty;
 
                 for (int rr = 0; rr < k; ++rr) {
-                    T xn[S];
-                    gather_next_rhs<T, S, KMAX>(x, cur, ks, rr, k, xn);
+                    xr.issue(x, cur, ks, rr, k, pairs);
                     T prod[S];
     #pragma unroll
                     for (int kk = 0; kk < S; ++kk) {
-                        const T pk = cur.v[kk] * xv[kk];
+                        const T pk = cur.v[kk] * xr.v[kk];
                         prod[kk] = tail ? keep_or_zero(pk, tile_item<T, S>(B, lane, kk) < mv.nnz) : pk;
                     }
                     // products -> LDS in tile order, back lane-contiguous
@@ -536,7 +581,7 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                         else if (lane == 0) mv.sync_val[(long long)w * KMAX + rr] = alpha * s_buf[0];
                     }
                     carry_set(rr, s_buf[K - 1]);
-                    next_rhs<T, S, KMAX>(rr, k, xn, xv);
+                    xr.advance(rr, k, pairs);
                 }
                 if (syncing) {
                     sync_key = mv.key_base + r0 * ks;

@@ -17,20 +17,20 @@ def say(msg):
     print(f"[gpu] {msg}", flush=True)
 
 
-def small_batch(seed, dtype, n=20):
+def small_batch(seed, dtype, n=20, k_choices=(1, 4, 16)):
     rng = np.random.default_rng(seed)
     mats, ks = [], []
     for i in range(n):
         A = matgen.family_matrix(seed * 1000 + i, 100, 4000, i % 4)
         mats.append(A)
-        ks.append(int(rng.choice([1, 4, 16])))
+        ks.append(int(rng.choice(k_choices)))
     # edge cases: no nonzeros, leading/trailing empty rows, one long row
     for c in (cases.from_lengths("empty", [0] * 7, seed=1),
               cases.from_lengths("lead/trail empty", [0] * 70 + [3] * 50 + [0] * 9, seed=2),
               cases.from_lengths("long row", [2, 5000, 3], n=5100, seed=3),
               cases.six_by_six()):
         mats.append(matgen.CSR(c.m, c.n, c.row_ptr, c.col, c.val, c.name))
-        ks.append(int(rng.choice([1, 4, 16])))
+        ks.append(int(rng.choice(k_choices)))
     Xs = [rng.uniform(-1, 1, (A.n, k)).astype(dtype) for A, k in zip(mats, ks)]
     return mats, ks, Xs
 
@@ -61,6 +61,27 @@ def test_batched_equals_plain_and_oracle(dtype, sched):
     for i, (yb, yp) in enumerate(zip(Yb, Yp)):
         assert np.array_equal(yb, yp), (i, mats[i].name)
     check_oracle(mats, ks, Xs, Yb, dtype)
+    b.close()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_odd_and_even_rhs_counts(dtype):
+    """Even right-hand-side counts take 2-wide x loads (from even x offsets),
+    odd ones one gather per RHS; matrices of both kinds share one batch.
+    Bit for bit against the plain launches, and within the oracle's bound."""
+    mats, ks, Xs = small_batch(13, dtype, n=16, k_choices=(2, 3, 5, 6, 7, 16))
+    say(f"odd/even k batch {np.dtype(dtype).name} ks={ks}")
+    b = batched.SpmvBatch(mats, ks, dtype)
+    for sched in ((8, 4, 4, 0, 1), (4, 2, 2, 0, 0)):
+        b.set_schedule(*sched)
+        b.set_x(Xs)
+        b.run(1.0)
+        Yb = b.y()
+        b.run_plain(1.0)
+        Yp = b.y(plain=True)
+        for i, (yb, yp) in enumerate(zip(Yb, Yp)):
+            assert np.array_equal(yb, yp), (sched, i, ks[i], mats[i].name)
+        check_oracle(mats, ks, Xs, Yb, dtype)
     b.close()
 
 
