@@ -35,6 +35,10 @@ class SpmvBatch:
         self._col = [np.ascontiguousarray(A.col if A.nnz else np.zeros(1), np.int32) for A in mats]
         self._val = [np.ascontiguousarray(A.val if A.nnz else np.zeros(1), self.dtype)
                      for A in mats]
+        for i, A in enumerate(mats):          # the library reads these many entries
+            if (len(self._rp[i]) < A.m + 1 or len(self._col[i]) < A.nnz
+                    or len(self._val[i]) < A.nnz or int(self._rp[i][A.m]) != A.nnz):
+                raise ValueError(f"matrix {i}: row_ptr/col/val shorter than m+1 / nnz")
         self.h = C.c_void_p()
         check(self._fn("batch_create")(C.byref(self.h), device, n, m.ctypes.data_as(C.c_void_p),
                                         nn.ctypes.data_as(C.c_void_p),

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <new>
 #include <string>
 #include <vector>
@@ -1323,6 +1324,24 @@ int row_features(bhs_handle_t h, double *f)
 
 #include "bhs_batch.inc"
 
+// Runs one entry point of the C ABI.  A failed host allocation
+// (std::bad_alloc from the std::vector work arrays of the analysis, the
+// panel layout or a batch) becomes BHSPARSE_OUT_OF_MEMORY: no C++ exception
+// crosses the C boundary, so the caller's process never aborts.
+template <typename F>
+int guard(F &&f)
+{
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        return set_error(BHSPARSE_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::exception &e) {
+        return set_error(BHSPARSE_HIP_ERROR, std::string("internal error: ") + e.what());
+    } catch (...) {
+        return set_error(BHSPARSE_HIP_ERROR, "internal error");
+    }
+}
+
 } // namespace
 
 extern "C" {
@@ -1403,59 +1422,59 @@ int bhs_stream_copy(int device, long long bytes, int reps, double *ms)
 }
 
 #define BHS_DEFINE_API(P, T)                                                                   \
-    int bhs_##P##_create(bhs_handle_t *h) { return create<T>(h); }                            \
-    int bhs_##P##_init_platform(bhs_handle_t h, int d) { return init_platform<T>(h, d); }    \
+    int bhs_##P##_create(bhs_handle_t *h) { return guard([&] { return create<T>(h); }); }                            \
+    int bhs_##P##_init_platform(bhs_handle_t h, int d) { return guard([&] { return init_platform<T>(h, d); }); }    \
     int bhs_##P##_init_data(bhs_handle_t h, int m, int n, int nnz, const int *rp,             \
                             const int *ci, const T *v, const T *x, T *y)                      \
     {                                                                                          \
-        return init_data<T>(h, m, n, nnz, rp, ci, v, x, y);                                    \
+        return guard([&] { return init_data<T>(h, m, n, nnz, rp, ci, v, x, y); });                                    \
     }                                                                                          \
-    int bhs_##P##_set_x(bhs_handle_t h, const T *x) { return set_x<T>(h, x); }                \
-    int bhs_##P##_set_memory_mode(bhs_handle_t h, int md) { return set_memory_mode<T>(h, md); } \
-    int bhs_##P##_get_memory_mode(bhs_handle_t h, int *md) { return get_memory_mode<T>(h, md); } \
-    int bhs_##P##_spmv(bhs_handle_t h, T a) { return spmv<T>(h, a); }                         \
-    int bhs_##P##_spmv_async(bhs_handle_t h, T a) { return spmv_async<T>(h, a); }             \
-    int bhs_##P##_get_stream(bhs_handle_t h, void **st) { return get_stream<T>(h, st); }      \
-    int bhs_##P##_get_y(bhs_handle_t h) { return get_y<T>(h); }                               \
-    int bhs_##P##_get_y_to(bhs_handle_t h, T *o) { return get_y_to<T>(h, o); }                \
-    int bhs_##P##_free_mem(bhs_handle_t h) { return free_mem<T>(h); }                         \
-    int bhs_##P##_free_platform(bhs_handle_t h) { return free_platform<T>(h); }               \
-    int bhs_##P##_destroy(bhs_handle_t h) { return destroy<T>(h); }                           \
-    int bhs_##P##_set_config(bhs_handle_t h, const bhs_config *c) { return set_config<T>(h, c); } \
-    int bhs_##P##_get_config(bhs_handle_t h, bhs_config *c) { return get_config<T>(h, c); }   \
+    int bhs_##P##_set_x(bhs_handle_t h, const T *x) { return guard([&] { return set_x<T>(h, x); }); }                \
+    int bhs_##P##_set_memory_mode(bhs_handle_t h, int md) { return guard([&] { return set_memory_mode<T>(h, md); }); } \
+    int bhs_##P##_get_memory_mode(bhs_handle_t h, int *md) { return guard([&] { return get_memory_mode<T>(h, md); }); } \
+    int bhs_##P##_spmv(bhs_handle_t h, T a) { return guard([&] { return spmv<T>(h, a); }); }                         \
+    int bhs_##P##_spmv_async(bhs_handle_t h, T a) { return guard([&] { return spmv_async<T>(h, a); }); }             \
+    int bhs_##P##_get_stream(bhs_handle_t h, void **st) { return guard([&] { return get_stream<T>(h, st); }); }      \
+    int bhs_##P##_get_y(bhs_handle_t h) { return guard([&] { return get_y<T>(h); }); }                               \
+    int bhs_##P##_get_y_to(bhs_handle_t h, T *o) { return guard([&] { return get_y_to<T>(h, o); }); }                \
+    int bhs_##P##_free_mem(bhs_handle_t h) { return guard([&] { return free_mem<T>(h); }); }                         \
+    int bhs_##P##_free_platform(bhs_handle_t h) { return guard([&] { return free_platform<T>(h); }); }               \
+    int bhs_##P##_destroy(bhs_handle_t h) { return guard([&] { return destroy<T>(h); }); }                           \
+    int bhs_##P##_set_config(bhs_handle_t h, const bhs_config *c) { return guard([&] { return set_config<T>(h, c); }); } \
+    int bhs_##P##_get_config(bhs_handle_t h, bhs_config *c) { return guard([&] { return get_config<T>(h, c); }); }   \
     int bhs_##P##_get_partition(bhs_handle_t h, int *t, int cap, int *np)                     \
     {                                                                                          \
-        return get_partition<T>(h, t, cap, np);                                                \
+        return guard([&] { return get_partition<T>(h, t, cap, np); });                                                \
     }                                                                                          \
-    int bhs_##P##_get_stats(bhs_handle_t h, bhs_spmv_stats *o) { return get_stats<T>(h, o); } \
+    int bhs_##P##_get_stats(bhs_handle_t h, bhs_spmv_stats *o) { return guard([&] { return get_stats<T>(h, o); }); } \
     int bhs_##P##_bench(bhs_handle_t h, T a, int w, int r, long long fb, double *mt,          \
                         double *mm)                                                            \
     {                                                                                          \
-        return bench<T>(h, a, w, r, fb, mt, mm);                                               \
+        return guard([&] { return bench<T>(h, a, w, r, fb, mt, mm); });                                               \
     }                                                                                          \
-    int bhs_##P##_row_features(bhs_handle_t h, double *f) { return row_features<T>(h, f); }    \
+    int bhs_##P##_row_features(bhs_handle_t h, double *f) { return guard([&] { return row_features<T>(h, f); }); }    \
     int bhs_##P##_bench_floor(bhs_handle_t h, int r, long long fb, double *ms)                \
     {                                                                                          \
-        return bench_floor<T>(h, r, fb, ms);                                                   \
+        return guard([&] { return bench_floor<T>(h, r, fb, ms); });                                                   \
     }                                                                                          \
     int bhs_##P##_bench_event_overhead(bhs_handle_t h, int r, long long fb, double *ms)       \
     {                                                                                          \
-        return bench_event_overhead<T>(h, r, fb, ms);                                          \
+        return guard([&] { return bench_event_overhead<T>(h, r, fb, ms); });                                          \
     }                                                                                          \
     int bhs_##P##_autotune(bhs_handle_t h, const bhs_config *c, int n, int r, long long fb,    \
                            double *us, int *best)                                              \
     {                                                                                          \
-        return autotune<T>(h, c, n, r, fb, us, best);                                          \
+        return guard([&] { return autotune<T>(h, c, n, r, fb, us, best); });                                          \
     }                                                                                          \
     int bhs_##P##_apply_tuner(bhs_handle_t h, bhs_mlp_t net, const bhs_config *c, int k,      \
                               int *chosen)                                                     \
     {                                                                                          \
-        return apply_tuner<T>(h, net, c, k, chosen);                                           \
+        return guard([&] { return apply_tuner<T>(h, net, c, k, chosen); });                                           \
     }                                                                                          \
     int bhs_##P##_apply_tuner_topk(bhs_handle_t h, bhs_mlp_t net, const bhs_config *c, int k, \
                                    int topk, int r, long long fb, int *chosen)                 \
     {                                                                                          \
-        return apply_tuner_topk<T>(h, net, c, k, topk, r, fb, chosen);                         \
+        return guard([&] { return apply_tuner_topk<T>(h, net, c, k, topk, r, fb, chosen); });                         \
     }
 
 BHS_DEFINE_API(s, float)
@@ -1466,34 +1485,34 @@ BHS_DEFINE_API(d, double)
                                const int *n, const int *nnz, const int *const *row_ptr,         \
                                const int *const *col, const T *const *val, const int *k)        \
     {                                                                                          \
-        return batch_create<T>(b, device, n_mats, m, n, nnz, row_ptr, col, val, k);            \
+        return guard([&] { return batch_create<T>(b, device, n_mats, m, n, nnz, row_ptr, col, val, k); });            \
     }                                                                                          \
-    int bhs_##P##_batch_set_x(bhs_batch_t b, const T *const *X) { return batch_set_x<T>(b, X); } \
+    int bhs_##P##_batch_set_x(bhs_batch_t b, const T *const *X) { return guard([&] { return batch_set_x<T>(b, X); }); } \
     int bhs_##P##_batch_set_schedule(bhs_batch_t b, const bhs_batch_schedule *s)               \
     {                                                                                          \
-        return batch_set_schedule<T>(b, s);                                                    \
+        return guard([&] { return batch_set_schedule<T>(b, s); });                                                    \
     }                                                                                          \
     int bhs_##P##_batch_get_schedule(bhs_batch_t b, bhs_batch_schedule *s)                     \
     {                                                                                          \
-        return batch_get_schedule<T>(b, s);                                                    \
+        return guard([&] { return batch_get_schedule<T>(b, s); });                                                    \
     }                                                                                          \
-    int bhs_##P##_batch_run(bhs_batch_t b, T alpha) { return batch_run<T>(b, alpha, 0); }     \
-    int bhs_##P##_batch_run_plain(bhs_batch_t b, T alpha) { return batch_run<T>(b, alpha, 1); } \
+    int bhs_##P##_batch_run(bhs_batch_t b, T alpha) { return guard([&] { return batch_run<T>(b, alpha, 0); }); }     \
+    int bhs_##P##_batch_run_plain(bhs_batch_t b, T alpha) { return guard([&] { return batch_run<T>(b, alpha, 1); }); } \
     int bhs_##P##_batch_get_y(bhs_batch_t b, T *const *Y, int plain)                           \
     {                                                                                          \
-        return batch_get_y<T>(b, Y, plain);                                                    \
+        return guard([&] { return batch_get_y<T>(b, Y, plain); });                                                    \
     }                                                                                          \
     int bhs_##P##_batch_bench(bhs_batch_t b, T alpha, int plain, int warmup, int reps,         \
                               double *ms)                                                      \
     {                                                                                          \
-        return batch_bench<T>(b, alpha, plain, warmup, reps, ms);                              \
+        return guard([&] { return batch_bench<T>(b, alpha, plain, warmup, reps, ms); });                              \
     }                                                                                          \
     int bhs_##P##_batch_autotune(bhs_batch_t b, const bhs_batch_schedule *c, int n, int reps, \
                                  double *ms, int *best)                                        \
     {                                                                                          \
-        return batch_autotune<T>(b, c, n, reps, ms, best);                                     \
+        return guard([&] { return batch_autotune<T>(b, c, n, reps, ms, best); });                                     \
     }                                                                                          \
-    int bhs_##P##_batch_destroy(bhs_batch_t b) { return batch_destroy<T>(b); }
+    int bhs_##P##_batch_destroy(bhs_batch_t b) { return guard([&] { return batch_destroy<T>(b); }); }
 
 BHS_DEFINE_BATCH_API(s, float)
 BHS_DEFINE_BATCH_API(d, double)

@@ -297,7 +297,7 @@ int bhs_tuner_transform(const double *raw8, float *out8)
 }
 
 int bhs_mlp_create(bhs_mlp_t *net, int device, int n_in, int n_hidden, int n_out)
-{
+try {
     if (!net || n_in != kTunerFeatures || n_hidden < 1 || n_hidden > kMaxHidden || n_out < 1 ||
         n_out > kMaxClasses)
         return mlp_set_error(BHSPARSE_INVALID_ARGUMENT, "network shape out of range");
@@ -331,6 +331,11 @@ int bhs_mlp_create(bhs_mlp_t *net, int device, int n_in, int n_hidden, int n_out
     *net = reinterpret_cast<bhs_mlp_t>(m);
     return BHSPARSE_SUCCESS;
 }
+catch (const std::bad_alloc &) {   // no exception crosses the C boundary
+    return mlp_set_error(BHSPARSE_OUT_OF_MEMORY, "host allocation failed");
+} catch (...) {
+    return mlp_set_error(BHSPARSE_HIP_ERROR, "internal error");
+}
 
 int bhs_mlp_destroy(bhs_mlp_t net)
 {
@@ -351,7 +356,7 @@ long long bhs_mlp_param_count(int n_in, int n_hidden, int n_out)
 
 int bhs_mlp_set_params(bhs_mlp_t net, const float *params, long long n, const float *norm_mean,
                        const float *norm_std)
-{
+try {
     Mlp *m;
     if (int rc = check_net(net, &m)) return rc;
     if (!params || n != m->n_params || !norm_mean || !norm_std)
@@ -366,6 +371,11 @@ int bhs_mlp_set_params(bhs_mlp_t net, const float *params, long long n, const fl
     MLP_HIP(hipMemcpy(m->d_norm, nm.data(), sizeof(float) * nm.size(), hipMemcpyHostToDevice));
     return BHSPARSE_SUCCESS;
 }
+catch (const std::bad_alloc &) {   // no exception crosses the C boundary
+    return mlp_set_error(BHSPARSE_OUT_OF_MEMORY, "host allocation failed");
+} catch (...) {
+    return mlp_set_error(BHSPARSE_HIP_ERROR, "internal error");
+}
 
 int bhs_mlp_get_params(bhs_mlp_t net, float *params, long long n)
 {
@@ -379,7 +389,7 @@ int bhs_mlp_get_params(bhs_mlp_t net, float *params, long long n)
 }
 
 int bhs_mlp_forward(bhs_mlp_t net, const float *x, int n_rows, float *out)
-{
+try {
     Mlp *m;
     if (int rc = check_net(net, &m)) return rc;
     if (!x || !out || n_rows < 1) return mlp_set_error(BHSPARSE_INVALID_ARGUMENT, "bad batch");
@@ -403,10 +413,15 @@ int bhs_mlp_forward(bhs_mlp_t net, const float *x, int n_rows, float *out)
     MLP_HIP(hipStreamSynchronize(m->stream));
     return BHSPARSE_SUCCESS;
 }
+catch (const std::bad_alloc &) {   // no exception crosses the C boundary
+    return mlp_set_error(BHSPARSE_OUT_OF_MEMORY, "host allocation failed");
+} catch (...) {
+    return mlp_set_error(BHSPARSE_HIP_ERROR, "internal error");
+}
 
 int bhs_mlp_loss_grad(bhs_mlp_t net, const float *x, const float *target, const float *mask,
                       int n_rows, double *loss, float *grad)
-{
+try {
     Mlp *m;
     if (int rc = check_net(net, &m)) return rc;
     if (!x || !target || !mask || !loss || !grad || n_rows < 1)
@@ -424,10 +439,15 @@ int bhs_mlp_loss_grad(bhs_mlp_t net, const float *x, const float *target, const 
     MLP_HIP(hipStreamSynchronize(m->stream));
     return BHSPARSE_SUCCESS;
 }
+catch (const std::bad_alloc &) {   // no exception crosses the C boundary
+    return mlp_set_error(BHSPARSE_OUT_OF_MEMORY, "host allocation failed");
+} catch (...) {
+    return mlp_set_error(BHSPARSE_HIP_ERROR, "internal error");
+}
 
 int bhs_mlp_train(bhs_mlp_t net, const float *x, const float *target, const float *mask,
                   int n_rows, int epochs, float lr, float weight_decay, double *loss_hist)
-{
+try {
     Mlp *m;
     if (int rc = check_net(net, &m)) return rc;
     if (!x || !target || !mask || n_rows < 1 || epochs < 1 || !(lr > 0))
@@ -458,11 +478,16 @@ int bhs_mlp_train(bhs_mlp_t net, const float *x, const float *target, const floa
     MLP_HIP(hipStreamSynchronize(m->stream));
     return BHSPARSE_SUCCESS;
 }
+catch (const std::bad_alloc &) {   // no exception crosses the C boundary
+    return mlp_set_error(BHSPARSE_OUT_OF_MEMORY, "host allocation failed");
+} catch (...) {
+    return mlp_set_error(BHSPARSE_HIP_ERROR, "internal error");
+}
 
 int bhs_mlp_loss_grad_f64(int device, int n_in, int n_hidden, int n_out, const double *params,
                           const double *norm, const double *x, const double *target,
                           const double *mask, int n_rows, double *loss, double *grad)
-{
+try {
     if (!params || !norm || !x || !target || !mask || !loss || !grad || n_rows < 1 ||
         n_in != kTunerFeatures || n_hidden < 1 || n_hidden > kMaxHidden || n_out < 1 ||
         n_out > kMaxClasses)
@@ -499,6 +524,11 @@ int bhs_mlp_loss_grad_f64(int device, int n_in, int n_hidden, int n_out, const d
     }
     (void)hipStreamDestroy(st);
     return rc;
+}
+catch (const std::bad_alloc &) {   // no exception crosses the C boundary
+    return mlp_set_error(BHSPARSE_OUT_OF_MEMORY, "host allocation failed");
+} catch (...) {
+    return mlp_set_error(BHSPARSE_HIP_ERROR, "internal error");
 }
 
 } // extern "C"

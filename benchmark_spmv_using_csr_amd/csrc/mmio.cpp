@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -51,9 +52,7 @@ bool next_data_line(FILE *f, std::string &line)
     return false;
 }
 
-} // namespace
-
-extern "C" int bhs_mm_read(const char *path, int flags, bhs_csr **out)
+int read_impl(const char *path, int flags, bhs_csr **out)
 {
     if (!path || !out) return BHSPARSE_INVALID_ARGUMENT;
     if ((flags & BHS_MM_MERGE_DUPLICATES) && !(flags & BHS_MM_SORT_COLUMNS))
@@ -88,13 +87,19 @@ extern "C" int bhs_mm_read(const char *path, int flags, bhs_csr **out)
         m > 0x7ffffffeLL || n > 0x7fffffffLL)
         return BHSPARSE_INVALID_MATRIX;
     if ((mirror || skew) && m != n) return BHSPARSE_INVALID_MATRIX;
+    // the declared count must fit the format: 32-bit indices, and no more
+    // entries than the matrix (or its stored triangle) has positions
+    const double positions = (mirror || skew) ? (double)m * (double)(m + 1) / 2 : (double)m * n;
+    if (nz > 0x7fffffffLL || (double)nz > positions) return BHSPARSE_INVALID_MATRIX;
 
-    // entries
+    // entries (storage grows with what the file holds, not with what its
+    // size line claims)
     std::vector<int> ri, ci;
     std::vector<double> vi;
-    ri.reserve((size_t)nz);
-    ci.reserve((size_t)nz);
-    vi.reserve((size_t)nz);
+    const size_t hint = (size_t)std::min<long long>(nz, 1LL << 16);
+    ri.reserve(hint);
+    ci.reserve(hint);
+    vi.reserve(hint);
     for (long long k = 0; k < nz; ++k) {
         if (!next_data_line(file.f, line)) return BHSPARSE_INVALID_MATRIX;
         long long i = 0, j = 0;
@@ -180,4 +185,19 @@ extern "C" int bhs_mm_read(const char *path, int flags, bhs_csr **out)
     std::copy(val.begin(), val.begin() + A->nnz, A->val);
     *out = A;
     return BHSPARSE_SUCCESS;
+}
+
+} // namespace
+
+// Host allocation failures (std::bad_alloc from the vectors) become an error
+// code: no exception crosses the C boundary.
+extern "C" int bhs_mm_read(const char *path, int flags, bhs_csr **out)
+{
+    try {
+        return read_impl(path, flags, out);
+    } catch (const std::bad_alloc &) {
+        return BHSPARSE_OUT_OF_MEMORY;
+    } catch (...) {
+        return BHSPARSE_INVALID_MATRIX;
+    }
 }

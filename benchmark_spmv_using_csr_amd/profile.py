@@ -44,6 +44,9 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--flush-mb", type=int, default=1024)
     ap.add_argument("--copy-mb", type=int, default=256)
+    ap.add_argument("--floor", action="store_true",
+                    help="also time the matrix's data-movement floor (floor_kernel) --reps times")
+    ap.add_argument("--no-spmv", action="store_true", help="skip the SpMV calls (with --floor)")
     a = ap.parse_args(argv)
     dtype = np.float64 if a.dtype == "f64" else np.float32
     A, x = workload(a.workload)
@@ -55,9 +58,14 @@ def main(argv=None):
     y = np.zeros(A.m, dtype)
     _lib.check(op.prepare_mem(A.m, A.n, A.nnz, A.row_ptr, A.col, A.val.astype(dtype),
                               x.astype(dtype), y), "prepare_mem")
-    t, k = op.bench(1.0, warmup=1, reps=a.reps, flush_bytes=a.flush_mb << 20)
-    print(f"spmv {A.name} {a.dtype}: median {np.median(t) * 1e3:.2f} us, main kernel mean "
-          f"{np.mean(k) * 1e3:.2f} us over {a.reps} calls", flush=True)
+    if not a.no_spmv:
+        t, k = op.bench(1.0, warmup=1, reps=a.reps, flush_bytes=a.flush_mb << 20)
+        print(f"spmv {A.name} {a.dtype}: median {np.median(t) * 1e3:.2f} us, main kernel mean "
+              f"{np.mean(k) * 1e3:.2f} us over {a.reps} calls", flush=True)
+    if a.floor:
+        f = op.bench_floor(reps=a.reps, flush_bytes=a.flush_mb << 20)
+        print(f"floor {A.name} {a.dtype}: median {np.median(f) * 1e3:.2f} us over {a.reps} calls",
+              flush=True)
     op.close()
     if a.copy_mb > 0:
         ms = stream_copy(a.copy_mb << 20, reps=a.reps)

@@ -53,6 +53,15 @@ class bhsparse_spmv:  # noqa: N801  (reference class name)
         ci = np.ascontiguousarray(col, np.int32)
         v = np.ascontiguousarray(val, self.dtype)
         xx = np.ascontiguousarray(x, self.dtype)
+        # the library reads row_ptr[0..m], col/val[0..nnz) and x[0..n) from
+        # these buffers: shorter ones would be read past their end
+        m, n, nnz = int(m), int(n), int(nnz)
+        if m < 0 or n < 0 or nnz < 0:
+            raise ValueError("m, n and nnz must be non-negative")
+        if len(rp) < m + 1 or len(ci) < nnz or len(v) < nnz or len(xx) < n:
+            raise ValueError(f"arrays shorter than the sizes: len(row_ptr)={len(rp)} < m+1="
+                             f"{m + 1}, or len(col)={len(ci)} / len(val)={len(v)} < nnz={nnz}, "
+                             f"or len(x)={len(xx)} < n={n}")
         if len(ci) == 0:
             ci = np.zeros(1, np.int32)
             v = np.zeros(1, self.dtype)
@@ -96,6 +105,8 @@ class bhsparse_spmv:  # noqa: N801  (reference class name)
 
     def set_x(self, x) -> int:
         xx = np.ascontiguousarray(x, self.dtype)
+        if len(xx) < self.n:
+            raise ValueError(f"x has {len(xx)} entries, the matrix has n={self.n} columns")
         if len(xx) == 0:
             xx = np.zeros(1, self.dtype)
         return self._fn("set_x")(self._h, xx)

@@ -28,7 +28,7 @@ _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 
 class TileStats(C.Structure):
     _fields_ = [(n, C.c_longlong) for n in
-                ("n_tiles", "n_waves", "n_fast", "n_dirty", "n_sync", "n_lead_empty", "max_chain")]
+                ("n_tiles", "n_waves", "n_fast", "n_dirty", "n_sync", "n_lead_empty")]
 
 
 class RefStats(C.Structure):
@@ -83,13 +83,15 @@ MUTATIONS = {
     "serial_drop_last": 1, "serial_sign": 2, "serial_f32_acc": 3, "serial_transpose": 4,
     "lower_bound": 5, "no_carry": 6, "no_span": 7, "no_calibration": 8, "no_fixup": 9,
 }
+# planted mistakes of the tuner's MLP backprop (caught by the finite-difference pin)
+MLP_MUTATIONS = {"mlp_no_mask": 10, "mlp_relu_grad": 11, "mlp_transpose": 12}
 
 
 class mutated:
     """Context manager: run the oracle with one planted mistake, then restore it."""
 
     def __init__(self, name):
-        self.kind = MUTATIONS[name]
+        self.kind = MUTATIONS[name] if name in MUTATIONS else MLP_MUTATIONS[name]
 
     def __enter__(self):
         lib().oracle_set_mutation(self.kind)

@@ -223,20 +223,10 @@ void oracle_tile_stats_compute(int m, int nnz, const int *row_ptr, int P, int T,
         for (int r = r0; r <= stop; r++)
             if (row_ptr[r] == row_ptr[r + 1]) { o->n_dirty++; break; }
     }
-    long long chain = 0;
     for (long long w = 0; w < NW; w++) {
         long long E0 = w * T * (long long)P;
         int rf = row_of(row_ptr, m, nnz, E0);
-        if ((long long)row_ptr[rf] < E0) {
-            o->n_sync++;
-            chain++;
-            if (chain > o->max_chain) o->max_chain = chain;
-            /* the chain continues while the next wave starts inside row rf */
-            long long E1 = (w + 1) * T * (long long)P;
-            if (E1 >= nnz || (long long)row_ptr[rf + 1] <= E1) chain = 0;
-        } else {
-            chain = 0;
-        }
+        if ((long long)row_ptr[rf] < E0) o->n_sync++;
     }
 }
 
@@ -485,7 +475,8 @@ double oracle_mlp_loss_grad(int n_in, int h, int n_out, const double *params,
             if (out) out[(size_t)n * n_out + k] = a;
             const double mk = M[(size_t)n * n_out + k], d = a - T[(size_t)n * n_out + k];
             loss += mk * d * d;
-            d3[k] = 2 * mk * d / msum;
+            /* planted mistake: the mask dropped from dLoss/dOut */
+            d3[k] = 2 * (g_mutation == ORACLE_MUT_MLP_NO_MASK ? 1.0 : mk) * d / msum;
         }
         if (!grad) continue;
         for (int k = 0; k < n_out; k++) {
@@ -495,7 +486,8 @@ double oracle_mlp_loss_grad(int n_in, int h, int n_out, const double *params,
         for (int i = 0; i < h; i++) {
             double a = 0;
             for (int k = 0; k < n_out; k++) a += d3[k] * W3[(size_t)k * h + i];
-            d2[i] = a2[i] > 0 ? a : 0;
+            /* planted mistake: the ReLU derivative ignored */
+            d2[i] = (a2[i] > 0 || g_mutation == ORACLE_MUT_MLP_RELU_GRAD) ? a : 0;
         }
         for (int j = 0; j < h; j++) {
             gb2[j] += d2[j];
@@ -503,7 +495,10 @@ double oracle_mlp_loss_grad(int n_in, int h, int n_out, const double *params,
         }
         for (int i = 0; i < h; i++) {
             double a = 0;
-            for (int j = 0; j < h; j++) a += d2[j] * W2[(size_t)j * h + i];
+            /* planted mistake: W2 transposed in the backward pass */
+            for (int j = 0; j < h; j++)
+                a += d2[j] * (g_mutation == ORACLE_MUT_MLP_TRANSPOSE ? W2[(size_t)i * h + j]
+                                                                      : W2[(size_t)j * h + i]);
             d1[i] = a1[i] > 0 ? a : 0;
         }
         for (int j = 0; j < h; j++) {

@@ -58,7 +58,7 @@ void oracle_partition(int m, int nnz, const int *row_ptr, int P, int *tile_row);
  *           bunch's first row through the synchronizer, kernels_cuda.h:98-102)
  *   n_lead_empty rows before rowOf(0) (never covered by a tile). */
 typedef struct {
-    long long n_tiles, n_waves, n_fast, n_dirty, n_sync, n_lead_empty, max_chain;
+    long long n_tiles, n_waves, n_fast, n_dirty, n_sync, n_lead_empty;
 } oracle_tile_stats;
 void oracle_tile_stats_compute(int m, int nnz, const int *row_ptr, int P, int T,
                                oracle_tile_stats *out);
@@ -94,7 +94,9 @@ double oracle_rel_error_f64(int m, const int *row_ptr, const int *col,
  * standardised with norm = {mean[n_in], std[n_in]}.  Returns the masked mean
  * squared error sum(M (O - T)^2) / sum(M) over X[N*n_in]; writes the outputs O
  * (N*n_out) and the gradient w.r.t. the parameters when the pointers are non-NULL.
- * No reference counterpart (the tuner is new); pinned by finite differences. */
+ * No reference counterpart (the tuner is new).  Pinned by central finite
+ * differences of the loss over every parameter and by a numpy matrix-product
+ * forward (tests/test_oracle.py). */
 double oracle_mlp_loss_grad(int n_in, int h, int n_out, const double *params,
                             const double *norm, const double *X, const double *T,
                             const double *M, int N, double *out, double *grad);
@@ -112,7 +114,10 @@ enum {
     ORACLE_MUT_NO_CARRY = 6,           /* segsum_ref: tile carry not added to item 0 */
     ORACLE_MUT_NO_SPAN = 7,            /* segsum_ref: lane-spanning partials dropped */
     ORACLE_MUT_NO_CALIBRATION = 8,     /* segsum_ref: synchronizer partials dropped */
-    ORACLE_MUT_NO_FIXUP = 9            /* segsum_ref: dirty tiles not de-speculated */
+    ORACLE_MUT_NO_FIXUP = 9,           /* segsum_ref: dirty tiles not de-speculated */
+    ORACLE_MUT_MLP_NO_MASK = 10,       /* mlp_loss_grad: mask dropped from dLoss/dOut */
+    ORACLE_MUT_MLP_RELU_GRAD = 11,     /* mlp_loss_grad: ReLU derivative ignored */
+    ORACLE_MUT_MLP_TRANSPOSE = 12      /* mlp_loss_grad: W2 transposed in backprop */
 };
 void oracle_set_mutation(int kind);
 

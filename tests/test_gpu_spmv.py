@@ -285,3 +285,31 @@ def test_bench_entry_point():
     assert np.median(k) <= np.median(t) * 1.05 + 0.002
     ov = op.bench_event_overhead(reps=5, flush_bytes=64 << 20)
     assert len(ov) == 5 and np.all(ov >= 0) and np.median(ov) < np.median(k)
+
+
+@pytest.mark.parametrize("x0", [np.inf, np.nan])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_non_finite_x0_stays_in_its_rows(x0, dtype):
+    # items past nnz in the last tile read col 0 / val 0; their products are
+    # masked, so an infinite or NaN x[0] reaches only the rows that hold column 0
+    # (fast-track tiles: the long rows; general and dirty tiles: the rest)
+    rng = np.random.default_rng(5)
+    lens = np.concatenate([rng.integers(0, 9, 3000), [4000, 2500], rng.integers(0, 5, 777)])
+    c = cases.from_lengths("x0 non-finite", lens, n=6000, seed=9)
+    assert c.nnz % 512 != 0 and c.nnz % 256 != 0
+    c.x = c.x.copy()
+    c.x[0] = x0
+    has0 = np.array([0 in c.col[c.row_ptr[i]:c.row_ptr[i + 1]] for i in range(c.m)])
+    assert has0.any() and (~has0).any()
+    v = c.val.astype(dtype).astype(np.float64)
+    xx = c.x.astype(dtype).astype(np.float64)
+    yr = oracle.spmv_serial(c.row_ptr, c.col, v, xx)
+    for variant, cfg, tpv in [(0, (8, 4, 3), 0), (0, (4, 1, 1), 0), (0, (16, 2, 5), 0),
+                              (2, (8, 1, 4), 4), (1, (8, 4, 1), 8)]:
+        say(f"x[0]={x0} variant={variant} cfg={cfg} {np.dtype(dtype).name}")
+        y, op = gpu_spmv(c, dtype, cfg, variant=variant, tpv=tpv)
+        op.close()
+        assert np.all(np.isfinite(y[~has0])), (variant, cfg)
+        assert not np.any(np.isfinite(y[has0])), (variant, cfg)
+        _, err = oracle.rel_error(c.row_ptr, c.col, v, xx, y, yr, per_row=True)
+        assert np.max(err[~has0]) <= TOL[dtype], (variant, cfg)

@@ -161,3 +161,71 @@ def test_openmp_equals_serial():
     assert t == 4 and np.array_equal(y1, y2)
     s, used = oracle.time_spmv(A.row_ptr, A.col, A.val, x, 1, 3)
     assert s > 0 and used == 1
+
+
+# ---- the tuner's MLP backprop (oracle_mlp_loss_grad; no reference counterpart) ----
+def _mlp_case(seed=0, n_in=8, h=6, n_out=5, N=7):
+    rng = np.random.default_rng(seed)
+    n_par = h * n_in + h + h * h + h + n_out * h + n_out
+    params = rng.normal(0, 0.7, n_par)
+    X = rng.normal(2.0, 3.0, (N, n_in))
+    norm = np.concatenate([X.mean(0) + 0.1, X.std(0) + 0.5])
+    T = rng.normal(0, 1, (N, n_out))
+    M = (rng.random((N, n_out)) < 0.6).astype(np.float64)   # a few masked targets
+    M[0, 0] = 1.0
+    return (n_in, h, n_out), params, norm, X, T, M
+
+
+def _mlp_forward_numpy(shape, params, norm, X):
+    # the same network as three dense matrix products (numpy), not the oracle's loops
+    n_in, h, n_out = shape
+    o = 0
+    W1 = params[o:o + h * n_in].reshape(h, n_in); o += h * n_in
+    b1 = params[o:o + h]; o += h
+    W2 = params[o:o + h * h].reshape(h, h); o += h * h
+    b2 = params[o:o + h]; o += h
+    W3 = params[o:o + n_out * h].reshape(n_out, h); o += n_out * h
+    b3 = params[o:o + n_out]
+    xs = (X - norm[:n_in]) / norm[n_in:]
+    a1 = np.maximum(xs @ W1.T + b1, 0)
+    a2 = np.maximum(a1 @ W2.T + b2, 0)
+    return a2 @ W3.T + b3, (xs @ W1.T + b1, a1 @ W2.T + b2)
+
+
+def _mlp_pins_hold(seed=0):
+    shape, params, norm, X, T, M = _mlp_case(seed)
+    loss, out, grad = oracle.mlp_loss_grad(*shape, params, norm, X, T, M)
+    ref_out, pre = _mlp_forward_numpy(shape, params, norm, X)
+    if np.max(np.abs(out - ref_out)) > 1e-12 * max(1.0, np.max(np.abs(ref_out))):
+        return False
+    if abs(loss - np.sum(M * (ref_out - T) ** 2) / np.sum(M)) > 1e-12 * max(1.0, loss):
+        return False
+    # central differences of the oracle's loss over every parameter; the step
+    # stays away from every ReLU kink (checked below), so the loss is smooth there
+    eps = 1e-6
+    assert min(np.min(np.abs(p)) for p in pre) > 1e-3, "case sits on a ReLU kink"
+    fd = np.empty_like(params)
+    for i in range(len(params)):
+        pp, pm = params.copy(), params.copy()
+        pp[i] += eps
+        pm[i] -= eps
+        lp = oracle.mlp_loss_grad(*shape, pp, norm, X, T, M)[0]
+        lm = oracle.mlp_loss_grad(*shape, pm, norm, X, T, M)[0]
+        fd[i] = (lp - lm) / (2 * eps)
+    scale = max(1e-3, np.max(np.abs(fd)))
+    return bool(np.max(np.abs(grad - fd)) <= 1e-6 * scale)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_mlp_backprop_matches_finite_differences(seed):
+    # the pin of oracle_mlp_loss_grad: its outputs against numpy matrix products,
+    # its gradient against central differences of its loss (8 -> 6 -> 6 -> 5,
+    # masked targets, relative tolerance 1e-6 of the largest gradient)
+    assert _mlp_pins_hold(seed)
+
+
+@pytest.mark.parametrize("mistake", sorted(oracle.MLP_MUTATIONS))
+def test_planted_mlp_mistake_breaks_the_pin(mistake):
+    with oracle.mutated(mistake):
+        assert not _mlp_pins_hold(0), mistake
+    assert _mlp_pins_hold(0)
