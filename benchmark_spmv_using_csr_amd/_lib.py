@@ -144,6 +144,7 @@ def exported_names():
     for p in ("s", "d"):
         names += [f"bhs_{p}_{k}" for k in _PER_TYPE]
         names += [f"bhs_{p}_{k}" for k in _BATCH]
+        names.append(f"bhs_{p}_spmv_noalloc")
     return names
 
 
@@ -178,6 +179,10 @@ def spmv_lib():
         f = getattr(L, name)
         f.argtypes = args
         f.restype = res
+    for p, vt in (("s", C.c_float), ("d", C.c_double)):
+        f = getattr(L, f"bhs_{p}_spmv_noalloc")   # device pointers, alpha, stream
+        f.argtypes = [C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 5 + [vt, C.c_void_p]
+        f.restype = C.c_int
     _spmv = L
     return L
 

@@ -15,11 +15,20 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
+
+// Device scratch of the allocation-free entry points (bhs_*_spmv_noalloc):
+// the tile partition and the waves' boundary-row partials, at most
+// kScratchBytes.  One copy per device, held until the process ends; host code
+// reaches it only through the address scratch_ptr() looks up.
+constexpr size_t kScratchBytes = size_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
 
@@ -160,6 +169,8 @@ int create(bhs_handle_t *h)
     return BHSPARSE_SUCCESS;
 }
 
+int scratch_ptr(void **out);
+
 template <typename T>
 int init_platform(bhs_handle_t h, int device)
 {
@@ -185,7 +196,8 @@ int init_platform(bhs_handle_t h, int device)
     }
     im->device = device;
     im->platform = true;
-    return BHSPARSE_SUCCESS;
+    void *sp = nullptr;              // the device's scratch address, looked up once
+    return scratch_ptr(&sp);
 }
 
 // Frees an array allocated by alloc_data in `mode`.
@@ -1324,6 +1336,106 @@ int row_features(bhs_handle_t h, double *f)
 
 #include "bhs_batch.inc"
 
+// ---- y = alpha*A*x with no allocation ------------------------------------
+// Device scratch of the allocation-free entry point: the tile partition and
+// the waves' boundary-row partials, at most kScratchBytes.  One copy per
+// device, held until the process ends.  Host code never uses the array's
+// name as an address: scratch_ptr() looks the device's copy up once per
+// device (hipGetSymbolAddress, with that device current) and keeps it.
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kScratchDevices] = {};
+
+// The current device's scratch (looked up on its first use on that device).
+int scratch_ptr(void **out)
+{
+    int dev = -1;
+    BHS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices)
+        return set_error(BHSPARSE_UNSUPPORTED_DEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        BHS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return set_error(BHSPARSE_HIP_ERROR, "hipGetSymbolAddress(scratch) gave null");
+        g_scratch_dev[dev] = p;
+    }
+    *out = g_scratch_dev[dev];
+    return BHSPARSE_SUCCESS;
+}
+
+// The same kernels as a handle's default tile (segmented sum, S = 8, 4 waves
+// per block, 4 tiles per wave, plain layout): partition, segmented sum with
+// checked loads of the last tile (the caller's col/val are not padded),
+// calibration.  Enqueued on `stream`; nothing is allocated, copied to the
+// host or waited for, so a caller may capture it in a graph.
+template <typename T>
+int spmv_noalloc(int m, int n, int nnz, const int *row_ptr, const int *col, const T *val,
+                 const T *x, T *y, T alpha, void *stream_v)
+{
+    constexpr int S = 8, WPB = 4, TPW = 4, P = 64 * S;
+    if (m < 0 || n < 0 || nnz < 0) return set_error(BHSPARSE_INVALID_ARGUMENT, "negative size");
+    if ((long long)nnz > kMaxNnz) return set_error(BHSPARSE_INVALID_MATRIX, "nnz too large");
+    if (m == 0) return BHSPARSE_SUCCESS;
+    if (!row_ptr || !y || (nnz > 0 && (!col || !val || !x)))
+        return set_error(BHSPARSE_INVALID_ARGUMENT, "null device pointer");
+    if (nnz > 0 && n == 0) return set_error(BHSPARSE_INVALID_MATRIX, "nonzeros in a matrix with n = 0");
+    if ((reinterpret_cast<uintptr_t>(col) | reinterpret_cast<uintptr_t>(val)) & 15)
+        return set_error(BHSPARSE_INVALID_ARGUMENT, "col and val must be 16-byte aligned");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream_v);
+    const long long NP = ((long long)nnz + P - 1) / P;
+    const long long NW = (NP + TPW - 1) / TPW;
+    // scratch: tile_row[NP+1] | sync_row[NW] | sync_val[NW], each 256-byte aligned
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t o_sr = up(sizeof(int) * (size_t)(NP + 1));
+    const size_t o_sv = o_sr + up(sizeof(int) * (size_t)std::max<long long>(NW, 1));
+    const size_t need = o_sv + sizeof(T) * (size_t)std::max<long long>(NW, 1);
+    if (need > kScratchBytes)
+        return set_error(BHSPARSE_INVALID_ARGUMENT,
+                         "matrix needs " + std::to_string(need) + " bytes of scratch, more than the " +
+                             std::to_string(kScratchBytes) + " the allocation-free entry point has");
+    void *base = nullptr;
+    if (int rc = scratch_ptr(&base)) return rc;
+    unsigned char *sb = static_cast<unsigned char *>(base);
+    int *tile_row = reinterpret_cast<int *>(sb);
+    int *sync_row = reinterpret_cast<int *>(sb + o_sr);
+    T *sync_val = reinterpret_cast<T *>(sb + o_sv);
+    const int threads = 256;
+    if (NP == 0) {                                    // no nonzeros: y = 0
+        const long long blocks = std::min<long long>((m + threads - 1) / threads, 4096);
+        hipLaunchKernelGGL(bhs::fill_kernel<T>, dim3((unsigned)blocks), dim3(threads), 0, st, y,
+                           (long long)m, T(0));
+        BHS_HIP(hipGetLastError());
+        return BHSPARSE_SUCCESS;
+    }
+    hipLaunchKernelGGL(bhs::partition_kernel, dim3((unsigned)((NP + 1 + threads - 1) / threads)),
+                       dim3(threads), 0, st, row_ptr, m, nnz, P, (int)NP, tile_row);
+    BHS_HIP(hipGetLastError());
+    bhs::SegsumParams<T> q;
+    q.m = m;
+    q.nnz = nnz;
+    q.NP = (int)NP;
+    q.TPW = TPW;
+    q.NW = (int)NW;
+    q.lead_empty = -1;                                // read from tile_row[0] on the device
+    q.wpb = WPB;
+    q.wave_lds_bytes = (256 + 64 * S * (int)sizeof(T) + 15) / 16 * 16;
+    q.alpha = alpha;
+    hipLaunchKernelGGL((bhs::segsum_kernel<T, S, false, 2, true>),
+                       dim3((unsigned)((NW + WPB - 1) / WPB)), dim3(64 * WPB),
+                       (size_t)WPB * q.wave_lds_bytes, st, row_ptr, col, val, x, y,
+                       (const int *)tile_row, sync_row, sync_val,
+                       (unsigned long long *)nullptr, q);
+    BHS_HIP(hipGetLastError());
+    if (NW > 1) {
+        hipLaunchKernelGGL(bhs::calibrate_kernel<T>, dim3((unsigned)((NW + threads - 1) / threads)),
+                           dim3(threads), 0, st, (const int *)sync_row, (const T *)sync_val,
+                           (int)NW, y);
+        BHS_HIP(hipGetLastError());
+    }
+    return BHSPARSE_SUCCESS;
+}
+
 // Runs one entry point of the C ABI.  A failed host allocation
 // (std::bad_alloc from the std::vector work arrays of the analysis, the
 // panel layout or a batch) becomes BHSPARSE_OUT_OF_MEMORY: no C++ exception
@@ -1479,6 +1591,18 @@ int bhs_stream_copy(int device, long long bytes, int reps, double *ms)
 
 BHS_DEFINE_API(s, float)
 BHS_DEFINE_API(d, double)
+
+int bhs_s_spmv_noalloc(int m, int n, int nnz, const int *row_ptr, const int *col, const float *val,
+                       const float *x, float *y, float alpha, void *stream)
+{
+    return guard([&] { return spmv_noalloc<float>(m, n, nnz, row_ptr, col, val, x, y, alpha, stream); });
+}
+
+int bhs_d_spmv_noalloc(int m, int n, int nnz, const int *row_ptr, const int *col,
+                       const double *val, const double *x, double *y, double alpha, void *stream)
+{
+    return guard([&] { return spmv_noalloc<double>(m, n, nnz, row_ptr, col, val, x, y, alpha, stream); });
+}
 
 #define BHS_DEFINE_BATCH_API(P, T)                                                             \
     int bhs_##P##_batch_create(bhs_batch_t *b, int device, int n_mats, const int *m,            \

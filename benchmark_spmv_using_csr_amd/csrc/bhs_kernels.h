@@ -92,6 +92,21 @@ __device__ __forceinline__ void tile_load(const int *__restrict__ col, const T *
     }
 }
 
+// The same items with every index checked against nnz (items past it read
+// as col 0 / val 0): the last tile of a matrix whose arrays are not padded.
+template <typename T, int S>
+__device__ __forceinline__ void tile_load_checked(const int *__restrict__ col,
+                                                  const T *__restrict__ val, int B, int lane,
+                                                  int nnz, Tile<T, S> &t)
+{
+#pragma unroll
+    for (int k = 0; k < S; ++k) {
+        const int e = tile_item<T, S>(B, lane, k);
+        t.c[k] = e < nnz ? col[e] : 0;
+        t.v[k] = e < nnz ? val[e] : T(0);
+    }
+}
+
 // Row boundaries of the first 64*CP rows of a tile starting at row r (CP = 2
 // or 4 chunk parts): lane l holds a[q] = row_ptr[r + 64q + l] for q < CP, and
 // every lane holds end = row_ptr[r + 64*CP], each index clamped to m (row_ptr
@@ -301,7 +316,9 @@ struct RhsX {
     }
 };
 
-template <typename T, int S, int KMAX, bool STRIDED, bool STATS, int CP>
+// BOUNDED: col/val hold exactly nnz items (no padding), so a tile reaching
+// past nnz is loaded with checked indices (a wave-uniform branch).
+template <typename T, int S, int KMAX, bool STRIDED, bool STATS, int CP, bool BOUNDED = false>
 __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, const int lane,
                                             unsigned *s_head, T *s_buf, T *s_carry, const T alpha,
                                             unsigned long long *stats)
@@ -360,7 +377,8 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
     // Row chunk of the next tile loaded one tile ahead; the next tile's stream
     // is in flight while this one is reduced (the last tile reloads itself).
     Tile<T, S> cur;
-    tile_load<T, S>(mv.col, mv.val, E0, lane, cur);
+    if (BOUNDED && E0 + P > mv.nnz) tile_load_checked<T, S>(mv.col, mv.val, E0, lane, mv.nnz, cur);
+    else tile_load<T, S>(mv.col, mv.val, E0, lane, cur);
     constexpr int CR = CP * kWave;              // rows a chunk covers
     RowChunk<CP> rc = load_chunk<CP>(row_ptr, r_first, lane, mv.m);
     int r1 = r_first;
@@ -376,7 +394,9 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
         rc = load_chunk<CP>(row_ptr, r1, lane, mv.m);
         __builtin_amdgcn_sched_barrier(0);
         Tile<T, S> nxt;
-        tile_load<T, S>(mv.col, mv.val, (p + 1 < p_end) ? B + P : B, lane, nxt);
+        const int Bn = (p + 1 < p_end) ? B + P : B;
+        if (BOUNDED && Bn + P > mv.nnz) tile_load_checked<T, S>(mv.col, mv.val, Bn, lane, mv.nnz, nxt);
+        else tile_load<T, S>(mv.col, mv.val, Bn, lane, nxt);
 
         const bool cont = read_lane(cc.a[0], 0) < B;   // item 0 continues row r0 (lane 0 holds row r0)
         if (p == p_begin) syncing = cont;
@@ -624,7 +644,9 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
     }
 }
 
-template <typename T, int S, bool STATS, int CP>
+// q.lead_empty < 0: read the number of leading empty rows from tile_row[0]
+// (the first tile's row), for launches that have not copied it to the host.
+template <typename T, int S, bool STATS, int CP, bool BOUNDED = false>
 __global__ __launch_bounds__(512) void segsum_kernel(
     const int *__restrict__ row_ptr, const int *__restrict__ col, const T *__restrict__ val,
     const T *__restrict__ x, T *__restrict__ y, const int *__restrict__ tile_row,
@@ -638,9 +660,10 @@ __global__ __launch_bounds__(512) void segsum_kernel(
     const int w = blk * q.wpb + wib;
     if (w >= q.NW) return;
     unsigned char *wave_lds = reinterpret_cast<unsigned char *>(smem_raw) + wib * q.wave_lds_bytes;
+    const int lead_empty = q.lead_empty >= 0 ? q.lead_empty : tile_row[0];
     MatView<T> mv{row_ptr, tile_row, col, val, x, y, sync_row, sync_val,
-                  q.m, q.nnz, q.NP, q.NW, q.lead_empty, 1, 1, q.TPW, 0};
-    segsum_wave<T, S, 1, false, STATS, CP>(mv, w, lane, reinterpret_cast<unsigned *>(wave_lds),
+                  q.m, q.nnz, q.NP, q.NW, lead_empty, 1, 1, q.TPW, 0};
+    segsum_wave<T, S, 1, false, STATS, CP, BOUNDED>(mv, w, lane, reinterpret_cast<unsigned *>(wave_lds),
                                        reinterpret_cast<T *>(wave_lds + 256), nullptr, q.alpha,
                                        stats);
 }

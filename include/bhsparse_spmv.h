@@ -247,6 +247,29 @@ int bhs_d_apply_tuner(bhs_handle_t h, bhs_mlp_t net, const bhs_config *classes, 
 int bhs_d_apply_tuner_topk(bhs_handle_t h, bhs_mlp_t net, const bhs_config *classes,
                            int n_classes, int topk, int reps, long long flush_bytes, int *chosen);
 
+/* ---- allocation-free SpMV ---------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): y = alpha * A * x over the caller's DEVICE arrays
+ * (row_ptr[m+1], col[nnz], val[nnz], x[n], y[m]; col and val 16-byte
+ * aligned), enqueued on `stream` (a hipStream_t, NULL = the null stream) and
+ * not waited for.  It runs the kernels of a handle's default tile (segmented
+ * sum, 8 items per lane, 4 waves per block, 4 tiles per wave, plain layout),
+ * so y equals bhs_*_spmv's with that tile bit for bit, with the tile
+ * partition and the cross-wave partials in a static device scratch array of
+ * 8 MiB per device (held until the process ends).  It calls no hipMalloc,
+ * hipFree, memset or copy.  A matrix needing more scratch than that
+ * ((nnz/512 + 1) * 4 bytes of partition plus (nnz/2048) * (4 + sizeof(T))
+ * bytes of partials, about 600M nonzeros in fp64) is refused with
+ * BHSPARSE_INVALID_ARGUMENT.  The caller guarantees a valid CSR (row_ptr
+ * monotone from 0 to nnz, 0 <= col < n): device data is not checked.
+ * Every call on a device shares its scratch: calls that overlap on one device
+ * (two host threads, or two streams) race and must be serialised. */
+int bhs_s_spmv_noalloc(int m, int n, int nnz, const int *row_ptr, const int *col,
+                       const float *val, const float *x, float *y, float alpha, void *stream);
+int bhs_d_spmv_noalloc(int m, int n, int nnz, const int *row_ptr, const int *col,
+                       const double *val, const double *x, double *y, double alpha, void *stream);
+
 /* ---- batched solves (no reference counterpart: the reference runs one
  * matrix and one right-hand side per process, spmv_cuda/main.cu:813-816) ---- */
 /* Copies n_mats CSR matrices (validated like init_data); matrix i gets k[i]
