@@ -158,6 +158,9 @@ def run_batched():
     ms, best = b.autotune(batched.SCHEDULES, reps=5)
     sched = batched.SCHEDULES[best]
     bt = b.bench(plain=False, warmup=2, reps=10)
+    # the same plain launches replayed from one hipGraph: what is left of the
+    # speedup without the per-launch overhead
+    pg = b.bench(plain=2, warmup=1, reps=5)
     b.run(1.0)
     yb = b.y()
     b.run_plain(1.0)
@@ -166,12 +169,21 @@ def run_batched():
     nnz = sum(A.nnz for A in mats)
     byts = sum(eff_bytes(A.m, A.n, A.nnz, 8, k) for A, k in zip(mats, ks))
     b.close()
+    ach = byts / (median(bt) * 1e-3) / 1e9
     return {
         "workload": "256 matrices, m log-uniform [1e3, 2e5], 4 families, k in {1,4,16}, fp64",
         "n_matrices": len(mats), "nnz": int(nnz), "rhs_total": int(sum(ks)),
         "plain_ms": round(median(plain), 4), "batched_ms": round(median(bt), 4),
         "speedup": round(median(plain) / median(bt), 3),
-        "batched_effective_gbps": round(byts / (median(bt) * 1e-3) / 1e9, 1),
+        "plain_graph_ms": round(median(pg), 4),
+        "speedup_vs_plain_graph": round(median(pg) / median(bt), 3),
+        "batched_effective_gbps": round(ach, 1),
+        "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
+                     "traffic": None,
+                     "definition": "effective bytes of the batch (A once, x and y per "
+                                   "right-hand side) / median graph replay (queue reset, "
+                                   "batched_kernel, calibration)"},
         "schedule": {"items_per_lane": sched[0], "tiles_per_wave": sched[1],
                      "waves_per_block": sched[2], "grid_blocks": sched[3],
                      "order": "matrix" if sched[4] == 0 else "most work first"},

@@ -105,7 +105,7 @@ class SpmvBatch:
 
 # Candidate schedules for the batch autotuner: (S, T, waves/block, grid, order).
 SCHEDULES = [(S, T, wpb, 0, 1) for S in (4, 8) for T in (1, 4, 16) for wpb in (1, 4)] + \
-            [(8, 4, 4, 0, 0)]
+            [(4, T, wpb, 0, 1) for T in (32, 64) for wpb in (1, 4)] + [(8, 4, 4, 0, 0)]
 
 
 def batch_set(seed=5, n_mats=256, m_lo=1e3, m_hi=2e5, dtype=np.float64):

@@ -112,6 +112,68 @@ __device__ __forceinline__ float dpp(float v)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 
+// Two right-hand sides carried together (the batched path): every operation
+// on a pair acts on each component exactly as on a single value, so
+// component c holds the bits a single-RHS pass would.
+typedef double double2v __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
+
+template <typename T, int R>
+struct RhsVec;
+template <typename T>
+struct RhsVec<T, 1> {
+    using type = T;
+};
+template <>
+struct RhsVec<double, 2> {
+    using type = double2v;
+};
+template <>
+struct RhsVec<float, 2> {
+    using type = float2v;
+};
+
+template <int CTRL>
+__device__ __forceinline__ double2v dpp(double2v v)
+{
+    return double2v{dpp<CTRL>(v.x), dpp<CTRL>(v.y)};
+}
+
+template <int CTRL>
+__device__ __forceinline__ float2v dpp(float2v v)
+{
+    return float2v{dpp<CTRL>(v.x), dpp<CTRL>(v.y)};
+}
+
+__device__ __forceinline__ double2v read_lane(double2v v, int l)
+{
+    return double2v{read_lane(v.x, l), read_lane(v.y, l)};
+}
+
+__device__ __forceinline__ float2v read_lane(float2v v, int l)
+{
+    return float2v{read_lane(v.x, l), read_lane(v.y, l)};
+}
+
+// lane i <- lane i-d (lanes below d keep their own value)
+template <typename T>
+__device__ __forceinline__ T shfl_up(T v, int d)
+{
+    return __shfl_up(v, d, kWave);
+}
+
+template <>
+__device__ __forceinline__ double2v shfl_up(double2v v, int d)
+{
+    return double2v{__shfl_up(v.x, d, kWave), __shfl_up(v.y, d, kWave)};
+}
+
+template <>
+__device__ __forceinline__ float2v shfl_up(float2v v, int d)
+{
+    return float2v{__shfl_up(v.x, d, kWave), __shfl_up(v.y, d, kWave)};
+}
+
 // Inclusive prefix sum of an int over the 64 lanes (DPP row shifts, then the
 // row broadcasts carry the running total across rows of 16).
 __device__ __forceinline__ int wave_incl_scan(int v, int /*lane*/)
@@ -152,7 +214,7 @@ __device__ __forceinline__ T wave_segscan(T v, uint64_t head, int lane)
     for (int d = 1; d < kWave; d <<= 1) {
         if (cov == ~0ull) break;
         // lane i <- lane i-d (lane 0 is a head and never uses its value)
-        T up = d == 1 ? dpp<kDppWaveShr1>(v) : __shfl_up(v, d, kWave);
+        T up = d == 1 ? dpp<kDppWaveShr1>(v) : shfl_up(v, d);
         if (!((cov >> lane) & 1ull)) v = up + v;
         cov |= cov << d;
     }
@@ -188,6 +250,16 @@ __device__ __forceinline__ float keep_or_zero(float v, bool keep)
 {
     int b = __builtin_bit_cast(int, v);
     return __builtin_bit_cast(float, keep ? b : 0);
+}
+
+__device__ __forceinline__ double2v keep_or_zero(double2v v, bool keep)
+{
+    return double2v{keep_or_zero(v.x, keep), keep_or_zero(v.y, keep)};
+}
+
+__device__ __forceinline__ float2v keep_or_zero(float2v v, bool keep)
+{
+    return float2v{keep_or_zero(v.x, keep), keep_or_zero(v.y, keep)};
 }
 
 // Block index remap so that each XCD (blocks are dealt round-robin over the

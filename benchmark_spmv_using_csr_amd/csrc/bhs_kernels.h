@@ -247,15 +247,11 @@ struct MatView {
 // x values of the tile's items for the right-hand side being reduced (v).
 // One right-hand side: a gather per item.  Several (KMAX > 1): the gathers
 // of the next right-hand side are issued before the current one is reduced,
-// so their latency overlaps that work.  When the stride between a column's
-// right-hand sides and their number are even, RHS rr and rr+1 (rr even) of
-// an item come from one aligned 2-wide load, half the gather requests; that
-// pair is issued while RHS rr-1 is reduced (RHS 0 and 1 with the tile's other
-// loads).  Each RHS's arithmetic is unchanged.
+// so their latency overlaps that work.
 template <typename T, int S, int KMAX>
 struct RhsX {
     T v[S];                                     // RHS rr
-    T n0[S], n1[S];                             // KMAX > 1: RHS rr+1, rr+2
+    T n0[S];                                    // KMAX > 1: RHS rr+1
 
     __device__ __forceinline__ void load1(const T *__restrict__ x, const Tile<T, S> &t, int ks,
                                           int rr, T (&d)[S])
@@ -263,57 +259,64 @@ struct RhsX {
 #pragma unroll
         for (int kk = 0; kk < S; ++kk) d[kk] = x[(long long)t.c[kk] * ks + rr];
     }
-    __device__ __forceinline__ void load2(const T *__restrict__ x, const Tile<T, S> &t, int ks,
-                                          int rr, T (&d0)[S], T (&d1)[S])
+    __device__ __forceinline__ void first(const T *__restrict__ x, const Tile<T, S> &t, int ks)
     {
-        using P2 = T __attribute__((ext_vector_type(2)));
-#pragma unroll
-        for (int kk = 0; kk < S; ++kk) {
-            const P2 q = *reinterpret_cast<const P2 *>(x + (long long)t.c[kk] * ks + rr);
-            // d1 first: load1 and load2 then both end with a store to d0, so
-            // the compiler's merge of the two branches never selects between
-            // two arrays (which would put them in scratch memory)
-            d1[kk] = q.y;
-            d0[kk] = q.x;
-        }
-    }
-    // RHS 0 (and RHS 1, into n0) of a tile
-    __device__ __forceinline__ void first(const T *__restrict__ x, const Tile<T, S> &t, int ks,
-                                          int k, bool pairs)
-    {
-        if constexpr (KMAX > 1) {
-            if (pairs && k > 1) {
-                load2(x, t, ks, 0, v, n0);
-                return;
-            }
-        }
         load1(x, t, ks, 0, v);
     }
-    // At the top of RHS rr: issue the loads of RHS rr+1 unless they are in
-    // n0 already (the second of a pair).  n0 receives RHS rr+1, n1 RHS rr+2.
-    // All branches are wave-uniform.
+    // at the top of RHS rr: the loads of RHS rr+1
     __device__ __forceinline__ void issue(const T *__restrict__ x, const Tile<T, S> &t, int ks,
-                                          int rr, int k, bool pairs)
+                                          int rr, int k)
     {
-        if constexpr (KMAX > 1) {
-            if (rr + 1 >= k) return;
-            if (!pairs) load1(x, t, ks, rr + 1, n0);
-            else if (rr & 1) load2(x, t, ks, rr + 1, n0, n1);   // (rr+1, rr+2); k is even
-        }
+        if constexpr (KMAX > 1)
+            if (rr + 1 < k) load1(x, t, ks, rr + 1, n0);
     }
-    // At the end of RHS rr: v becomes RHS rr+1, and after a pair n0 its second.
-    __device__ __forceinline__ void advance(int rr, int k, bool pairs)
+    // at the end of RHS rr: v becomes RHS rr+1
+    __device__ __forceinline__ void advance(int rr, int k)
     {
         if constexpr (KMAX > 1) {
             if (rr + 1 >= k) return;
 #pragma unroll
             for (int kk = 0; kk < S; ++kk) v[kk] = n0[kk];
-            if (pairs && (rr & 1)) {
-#pragma unroll
-                for (int kk = 0; kk < S; ++kk) n0[kk] = n1[kk];
-            }
         }
     }
+};
+
+// Right-hand sides in pairs (k even, even stride, x offsets even): RHS rr and
+// rr+1 of an item come from one aligned 2-wide load and are reduced together
+// as 2-vectors, each component with exactly the single-RHS arithmetic; the
+// next pair's loads are in flight while the current pair is reduced.
+template <typename T, int S>
+struct RhsPair {
+    using V = typename RhsVec<T, 2>::type;
+    V v[S], n[S];
+
+    __device__ __forceinline__ void load(const T *__restrict__ x, const Tile<T, S> &t, int ks,
+                                         int rr, V (&d)[S])
+    {
+#pragma unroll
+        for (int kk = 0; kk < S; ++kk)
+            d[kk] = *reinterpret_cast<const V *>(x + (long long)t.c[kk] * ks + rr);
+    }
+    __device__ __forceinline__ void first(const T *__restrict__ x, const Tile<T, S> &t, int ks)
+    {
+        load(x, t, ks, 0, v);
+    }
+    __device__ __forceinline__ void issue(const T *__restrict__ x, const Tile<T, S> &t, int ks,
+                                          int rr, int k)
+    {
+        if (rr + 2 < k) load(x, t, ks, rr + 2, n);
+    }
+    __device__ __forceinline__ void advance(int rr, int k)
+    {
+        if (rr + 2 >= k) return;
+#pragma unroll
+        for (int kk = 0; kk < S; ++kk) v[kk] = n[kk];
+    }
+};
+
+template <int R>
+struct RhsTag {
+    static constexpr int value = R;
 };
 
 // BOUNDED: col/val hold exactly nnz items (no padding), so a tile reaching
@@ -327,9 +330,11 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
     constexpr int G = Tile<T, S>::G;
     const int k = (KMAX == 1) ? 1 : mv.k;
     const int ks = STRIDED ? mv.kstride : 1;
-    // 2-wide x loads: whole pairs of right-hand sides (k even) at an even
-    // stride, from x offsets that keep them aligned
-    const bool pairs = KMAX > 1 && (ks & 1) == 0 && (k & 1) == 0;
+    // right-hand sides in pairs: k even at an even stride, from x and y
+    // offsets that keep the 2-wide loads and stores aligned; only where a
+    // tile's pairs fit the registers (S * sizeof(T) <= 32 bytes per lane)
+    constexpr bool kPairs = KMAX > 1 && S * sizeof(T) <= 32;
+    const bool pairs = kPairs && (ks & 1) == 0 && (k & 1) == 0;
     const int *__restrict__ row_ptr = mv.row_ptr;
     const int *__restrict__ tile_row = mv.tile_row;
     const T *__restrict__ x = mv.x;
@@ -365,10 +370,19 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
         if constexpr (KMAX == 1) return carry1;
         else return s_carry[rr];
     };
-    auto carry_set = [&](int rr, T v) {
+    // the open row's partial of RHS rr .. rr+R-1 as a value (R = 1) or pair
+    auto carry_getv = [&](auto rt, int rr) {
+        constexpr int R = decltype(rt)::value;
+        using V = typename RhsVec<T, R>::type;
+        if constexpr (R == 1) return carry_get(rr);
+        else return *reinterpret_cast<const V *>(s_carry + rr);
+    };
+    auto carry_setv = [&](auto rt, int rr, typename RhsVec<T, decltype(rt)::value>::type v) {
+        constexpr int R = decltype(rt)::value;
+        using V = typename RhsVec<T, R>::type;
         if constexpr (KMAX == 1) carry1 = v;
         else {
-            if (lane == 0) s_carry[rr] = v;
+            if (lane == 0) *reinterpret_cast<V *>(s_carry + rr) = v;
             wave_sync();
         }
     };
@@ -389,7 +403,13 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
         r1 = tile_row[p + 1];
         const bool general = r0 != r1;
         RhsX<T, S, KMAX> xr;
-        xr.first(x, cur, ks, k, pairs);
+        [[maybe_unused]] RhsPair<T, S> xp;
+        if constexpr (kPairs) {
+            if (pairs) xp.first(x, cur, ks);
+            else xr.first(x, cur, ks);
+        } else {
+            xr.first(x, cur, ks);
+        }
         const RowChunk<CP> cc = rc;
         rc = load_chunk<CP>(row_ptr, r1, lane, mv.m);
         __builtin_amdgcn_sched_barrier(0);
@@ -402,22 +422,42 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
         if (p == p_begin) syncing = cont;
         const bool tail = B + P > mv.nnz;
 
-        if (!general) {                          // fast track
-            for (int rr = 0; rr < k; ++rr) {
-                xr.issue(x, cur, ks, rr, k, pairs);
-                T prod[S];
+        // products of the tile for RHS rr .. rr+R-1 (items past nnz masked)
+        auto products = [&](auto rt, const auto &xv, auto (&prod)[S]) {
 #pragma unroll
-                for (int kk = 0; kk < S; ++kk) {
-                    const T pk = cur.v[kk] * xr.v[kk];
-                    prod[kk] = tail ? keep_or_zero(pk, tile_item<T, S>(B, lane, kk) < mv.nnz) : pk;
-                }
-                T s = prod[0];
+            for (int kk = 0; kk < S; ++kk) {
+                const auto pk = cur.v[kk] * xv[kk];
+                prod[kk] = tail ? keep_or_zero(pk, tile_item<T, S>(B, lane, kk) < mv.nnz) : pk;
+            }
+        };
+        if (!general) {                          // fast track
+            auto fast_rhs = [&](auto rt, int rr, const auto &xv) __attribute__((always_inline)) {
+                using V = typename RhsVec<T, decltype(rt)::value>::type;
+                V prod[S];
+                products(rt, xv, prod);
+                V s = prod[0];
 #pragma unroll
                 for (int kk = 1; kk < S; ++kk) s += prod[kk];
                 s = wave_sum(s);
-                carry_set(rr, cont ? carry_get(rr) + s : s);
-                xr.advance(rr, k, pairs);
+                carry_setv(rt, rr, cont ? carry_getv(rt, rr) + s : s);
+            };
+            bool done = false;
+            if constexpr (kPairs) {
+                if (pairs) {
+                    for (int rr = 0; rr < k; rr += 2) {
+                        xp.issue(x, cur, ks, rr, k);
+                        fast_rhs(RhsTag<2>{}, rr, xp.v);
+                        xp.advance(rr, k);
+                    }
+                    done = true;
+                }
             }
+            if (!done)
+                for (int rr = 0; rr < k; ++rr) {
+                    xr.issue(x, cur, ks, rr, k);
+                    fast_rhs(RhsTag<1>{}, rr, xr.v);
+                    xr.advance(rr, k);
+                }
             open = true;
             if (STATS) ++n_fast;
         } else {
@@ -512,55 +552,57 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
 
                 if (STATS && dirty) ++n_dirty;
 
-                for (int rr = 0; rr < k; ++rr) {
-                    xr.issue(x, cur, ks, rr, k, pairs);
-                    T prod[S];
-    #pragma unroll
-                    for (int kk = 0; kk < S; ++kk) {
-                        const T pk = cur.v[kk] * xr.v[kk];
-                        prod[kk] = tail ? keep_or_zero(pk, tile_item<T, S>(B, lane, kk) < mv.nnz) : pk;
-                    }
+                auto general_rhs = [&](auto rt, int rr, const auto &xv) __attribute__((always_inline)) {
+                    constexpr int R = decltype(rt)::value;
+                    using V = typename RhsVec<T, R>::type;
+                    V *sb = reinterpret_cast<V *>(s_buf);
+                    const V zero = static_cast<V>(T(0));
+                    auto store_y = [&](long long row, V val) {
+                        *reinterpret_cast<V *>(y + row * ks + rr) = val;
+                    };
+                    V prod[S];
+                    products(rt, xv, prod);
                     // products -> LDS in tile order, back lane-contiguous
                     wave_sync();
     #pragma unroll
                     for (int j = 0; j < S / G; ++j) {
-                        T *dst = s_buf + j * (kWave * G) + lane * G;
+                        V *dst = sb + j * (kWave * G) + lane * G;
     #pragma unroll
                         for (int g = 0; g < G; ++g) dst[g] = prod[j * G + g];
                     }
                     wave_sync();
-                    T pc[S];
+                    V pc[S];
     #pragma unroll
-                    for (int kk = 0; kk < S; ++kk) pc[kk] = s_buf[lane * S + kk];
+                    for (int kk = 0; kk < S; ++kk) pc[kk] = sb[lane * S + kk];
                     wave_sync();
-                    if (lane == 0 && cont) pc[0] = pc[0] + carry_get(rr);
+                    if (lane == 0 && cont) pc[0] = pc[0] + carry_getv(rt, rr);
 
                     // ---- lane-sequential segmented reduction -----------------
-                    T acc = T(0), lead = T(0);
+                    V acc = zero, lead = zero;
                     bool seen = false;
                     int sidx = seg_base;
     #pragma unroll
                     for (int kk = 0; kk < S; ++kk) {
                         if ((hb >> kk) & 1u) {
-                            if (seen) { s_buf[sidx] = acc; ++sidx; }
+                            if (seen) { sb[sidx] = acc; ++sidx; }
                             else { lead = acc; seen = true; }
-                            acc = T(0);
+                            acc = zero;
                         }
                         acc += pc[kk];
                     }
 
                     // ---- segments spanning lanes -----------------------------
                     const uint64_t present = ballot(seen);  // bit 0 always set
-                    const T I = wave_segscan(acc, present, lane);
-                    const T Iprev = dpp<kDppWaveShr1>(I);     // lane i <- lane i-1 (lane 0: unused)
-                    if (seen && lane > 0) s_buf[seg_base - 1] = Iprev + lead;
-                    if (lane == kWave - 1) s_buf[K - 1] = I;
+                    const V I = wave_segscan(acc, present, lane);
+                    const V Iprev = dpp<kDppWaveShr1>(I);     // lane i <- lane i-1 (lane 0: unused)
+                    if (seen && lane > 0) sb[seg_base - 1] = Iprev + lead;
+                    if (lane == kWave - 1) sb[K - 1] = I;
                     wave_sync();
 
                     // ---- output ------------------------------------------------
                     if (!dirty) {
                         for (int kk = kb + lane; kk < ke; kk += kWave)
-                            y[(long long)(r0 + kk) * ks + rr] = alpha * s_buf[kk];
+                            store_y(r0 + kk, alpha * sb[kk]);
                     } else {
                         // a nonempty row's segment is its rank among the
                         // tile's nonempty rows; an empty row gets 0
@@ -572,9 +614,9 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                             const bool ne = (mne[q] >> lane) & 1ull;
                             const int rank = running + popc_below(mne[q]);
                             if (ne && rank >= kb && rank < ke)
-                                y[(long long)row * ks + rr] = alpha * s_buf[rank];
+                                store_y(row, alpha * sb[rank]);
                             else if (row <= stop && !ne)
-                                y[(long long)row * ks + rr] = T(0);
+                                store_y(row, zero);
                             running += __popcll(mne[q]);
                         });
                         if constexpr (BIG)
@@ -591,18 +633,36 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                             const uint64_t msk = ballot(ne);
                             const int rank = running + popc_below(msk);
                             if (ne && rank >= kb && rank < ke)
-                                y[(long long)r * ks + rr] = alpha * s_buf[rank];
+                                store_y(r, alpha * sb[rank]);
                             running += __popcll(msk);
                             rb += kWave;
                         }
                     }
                     if (syncing) {
                         if constexpr (KMAX == 1) sync_v1 = alpha * s_buf[0];
-                        else if (lane == 0) mv.sync_val[(long long)w * KMAX + rr] = alpha * s_buf[0];
+                        else if (lane == 0)
+                            *reinterpret_cast<V *>(mv.sync_val + (long long)w * KMAX + rr) =
+                                alpha * sb[0];
                     }
-                    carry_set(rr, s_buf[K - 1]);
-                    xr.advance(rr, k, pairs);
+                    carry_setv(rt, rr, sb[K - 1]);
+                };
+                bool done = false;
+                if constexpr (kPairs) {
+                    if (pairs) {
+                        for (int rr = 0; rr < k; rr += 2) {
+                            xp.issue(x, cur, ks, rr, k);
+                            general_rhs(RhsTag<2>{}, rr, xp.v);
+                            xp.advance(rr, k);
+                        }
+                        done = true;
+                    }
                 }
+                if (!done)
+                    for (int rr = 0; rr < k; ++rr) {
+                        xr.issue(x, cur, ks, rr, k);
+                        general_rhs(RhsTag<1>{}, rr, xr.v);
+                        xr.advance(rr, k);
+                    }
                 if (syncing) {
                     sync_key = mv.key_base + r0 * ks;
                     syncing = false;
@@ -804,8 +864,8 @@ __global__ __launch_bounds__(512) void batched_kernel(
     const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     unsigned char *wave_lds = reinterpret_cast<unsigned char *>(smem_raw) + wib * wave_lds_bytes;
     unsigned *s_head = reinterpret_cast<unsigned *>(wave_lds);
-    T *s_buf = reinterpret_cast<T *>(wave_lds + 256);
-    T *s_carry = s_buf + kWave * S;
+    T *s_buf = reinterpret_cast<T *>(wave_lds + 256);   // room for 64*S pairs
+    T *s_carry = s_buf + 2 * kWave * S;
     for (;;) {
         int it = 0;
         if (lane == 0) it = atomicAdd(queue, 1);

@@ -47,7 +47,21 @@ def main(argv=None):
     ap.add_argument("--floor", action="store_true",
                     help="also time the matrix's data-movement floor (floor_kernel) --reps times")
     ap.add_argument("--no-spmv", action="store_true", help="skip the SpMV calls (with --floor)")
+    ap.add_argument("--batched", default="",
+                    help="S,T,waves_per_block,grid,order: run the 256-matrix batch (fp64) with "
+                         "this schedule --reps times instead of the single-matrix workload")
     a = ap.parse_args(argv)
+    if a.batched:
+        from . import batched
+        mats, ks, Xs = batched.batch_set()
+        b = batched.SpmvBatch(mats, ks, np.float64)
+        b.set_schedule(*(int(v) for v in a.batched.split(",")))
+        b.set_x(Xs)
+        ms = b.bench(plain=False, warmup=1, reps=a.reps)
+        print(f"batched {len(mats)} matrices, {sum(ks)} right-hand sides, schedule {a.batched}: "
+              f"median {np.median(ms):.4f} ms over {a.reps} graph replays", flush=True)
+        b.close()
+        return 0
     dtype = np.float64 if a.dtype == "f64" else np.float32
     A, x = workload(a.workload)
     op = bhsparse_spmv(dtype)

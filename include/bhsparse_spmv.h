@@ -290,7 +290,9 @@ int bhs_d_batch_run(bhs_batch_t b, double alpha);
 int bhs_d_batch_run_plain(bhs_batch_t b, double alpha);
 /* Y[i] (m[i] x k[i], row-major) from the batched (plain = 0) or baseline run. */
 int bhs_d_batch_get_y(bhs_batch_t b, double *const *Y, int plain);
-/* HIP-event times (ms) of `reps` batched (plain = 0) or baseline runs. */
+/* HIP-event times (ms) of `reps` runs: plain = 0 the batched graph, 1 the
+ * baseline's launches, 2 the baseline's launches replayed as one hipGraph
+ * (its kernels without the per-launch overhead). */
 int bhs_d_batch_bench(bhs_batch_t b, double alpha, int plain, int warmup, int reps, double *ms);
 /* Times every candidate schedule (median of reps), installs the fastest. */
 int bhs_d_batch_autotune(bhs_batch_t b, const bhs_batch_schedule *cands, int n_cands, int reps,

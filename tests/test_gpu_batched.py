@@ -66,9 +66,11 @@ def test_batched_equals_plain_and_oracle(dtype, sched):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_odd_and_even_rhs_counts(dtype):
-    """Even right-hand-side counts take 2-wide x loads (from even x offsets),
-    odd ones one gather per RHS; matrices of both kinds share one batch.
-    Bit for bit against the plain launches, and within the oracle's bound."""
+    """Even right-hand-side counts are reduced in pairs (2-wide x loads from
+    even x offsets, 2-wide y stores to even y offsets) where the tile allows
+    (S=4 in fp64, S<=8 in fp32), odd ones one RHS at a time; matrices of both
+    kinds share one batch.  Bit for bit against the plain launches, and
+    within the oracle's bound."""
     mats, ks, Xs = small_batch(13, dtype, n=16, k_choices=(2, 3, 5, 6, 7, 16))
     say(f"odd/even k batch {np.dtype(dtype).name} ks={ks}")
     b = batched.SpmvBatch(mats, ks, dtype)
@@ -132,4 +134,20 @@ def test_full_256_matrix_batch():
     Yp = b.y(plain=True)
     assert all(np.array_equal(g, r) for g, r in zip(Yb, Yp))
     check_oracle(mats, ks, Xs, Yb, np.float64, every=16)
+    b.close()
+
+
+def test_plain_graph_replay_matches_plain_launches():
+    """The baseline's launches replayed as one hipGraph (bench mode 2) give the
+    plain launches' y, every replay."""
+    mats, ks, Xs = small_batch(21, np.float64, n=10)
+    b = batched.SpmvBatch(mats, ks, np.float64)
+    b.set_schedule(4, 4, 2, 0, 1)
+    b.set_x(Xs)
+    b.run_plain(1.0)
+    Yp = b.y(plain=True)
+    for _ in range(2):
+        b.bench(plain=2, warmup=0, reps=2)
+        for i, (u, v) in enumerate(zip(b.y(plain=True), Yp)):
+            assert np.array_equal(u, v), i
     b.close()
