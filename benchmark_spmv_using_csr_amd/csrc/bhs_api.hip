@@ -8,6 +8,7 @@
 // put in managed mode, nothing is computed on the host per call, and a call
 // never depends on y's previous contents.
 #include <hip/hip_ext.h>
+#include <hipcub/hipcub.hpp>
 
 #include "bhs_kernels.h"
 #include "bhs_tuner.h"
@@ -315,9 +316,103 @@ Layout<T> active_layout(Impl<T> *im)
     return {im->m, im->d_row_ptr, im->d_col, im->d_val, im->d_y};
 }
 
-// Column-panel copy of A: a host pass over the device copy (counting sort by
-// (panel, row)).  Only nonempty (panel, row) pairs become stacked rows, so the
-// stacked matrix has no empty rows; columns stay sorted inside every one.
+// Column-panel copy of A, built on the device from its device copy: per row
+// the entries in every panel (panel_count_kernel), exclusive sums over the
+// (panel, row) pairs in panel-major order of the nonempty flags (the stacked
+// row of a pair) and of the counts (its first entry), then the entries
+// scattered in order (panel_scatter_kernel).  Only nonempty (panel, row) pairs
+// become stacked rows, so the stacked matrix has no empty rows; columns keep
+// their order inside every one.  The host keeps the stacked row_ptr, the row
+// of A of every stacked row and each panel's first stacked row, for the chain
+// records of the combine.
+struct IsPositive {
+    __host__ __device__ int operator()(int v) const { return v > 0 ? 1 : 0; }
+};
+
+template <typename T, int NPAN>
+int build_panels_n(Impl<T> *im)
+{
+    const int m = im->m, n = im->n, nnz = im->nnz;
+    const int pw = std::max((n + NPAN - 1) / NPAN, 1);
+    const long long PM = (long long)NPAN * m;
+    const long long G = ((long long)m + 63) / 64;
+    const int threads = 256;
+    int *d_cnt = nullptr, *d_rows = nullptr, *d_nzs = nullptr, *d_owner = nullptr;
+    void *d_tmp = nullptr;
+    // frees the work arrays on every return path
+    struct Work {
+        int **a[4];
+        void **t;
+        ~Work()
+        {
+            for (int **p : a)
+                if (*p) (void)hipFree(*p);
+            if (*t) (void)hipFree(*t);
+        }
+    } work{{&d_cnt, &d_rows, &d_nzs, &d_owner}, &d_tmp};
+    BHS_HIP(hipMalloc(&d_cnt, sizeof(int) * (size_t)PM));
+    BHS_HIP(hipMalloc(&d_rows, sizeof(int) * (size_t)PM));
+    BHS_HIP(hipMalloc(&d_nzs, sizeof(int) * (size_t)PM));
+    BHS_HIP(hipMalloc(&im->d_pmask, std::max<size_t>((size_t)m, 1)));
+    BHS_HIP(hipMalloc(&im->d_pbase, sizeof(int) * (size_t)std::max<long long>(G * NPAN, 1)));
+    const unsigned row_blocks = (unsigned)(((long long)m + threads / 64 - 1) / (threads / 64));
+    hipLaunchKernelGGL(bhs::panel_count_kernel<NPAN>, dim3(row_blocks),
+                       dim3(threads), 0, im->stream, (const int *)im->d_row_ptr,
+                       (const int *)im->d_col, m, pw, d_cnt, im->d_pmask);
+    BHS_HIP(hipGetLastError());
+    hipcub::TransformInputIterator<int, IsPositive, const int *> flags(d_cnt, IsPositive());
+    size_t tb1 = 0, tb2 = 0;
+    BHS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb1, flags, d_rows, (int)PM, im->stream));
+    BHS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb2, (const int *)d_cnt, d_nzs, (int)PM,
+                                             im->stream));
+    size_t tb = std::max(tb1, tb2);
+    BHS_HIP(hipMalloc(&d_tmp, std::max<size_t>(tb, 16)));
+    BHS_HIP(hipcub::DeviceScan::ExclusiveSum(d_tmp, tb, flags, d_rows, (int)PM, im->stream));
+    tb = std::max(tb1, tb2);
+    BHS_HIP(hipcub::DeviceScan::ExclusiveSum(d_tmp, tb, (const int *)d_cnt, d_nzs, (int)PM,
+                                             im->stream));
+    int last[2] = {0, 0};
+    BHS_HIP(hipMemcpyAsync(&last[0], d_rows + PM - 1, sizeof(int), hipMemcpyDeviceToHost, im->stream));
+    BHS_HIP(hipMemcpyAsync(&last[1], d_cnt + PM - 1, sizeof(int), hipMemcpyDeviceToHost, im->stream));
+    std::vector<int> pan_start((size_t)NPAN + 1, 0);
+    for (int p = 0; p < NPAN; ++p)
+        BHS_HIP(hipMemcpyAsync(&pan_start[(size_t)p], d_rows + (long long)p * m, sizeof(int),
+                               hipMemcpyDeviceToHost, im->stream));
+    BHS_HIP(hipStreamSynchronize(im->stream));
+    const int M = last[0] + (last[1] > 0 ? 1 : 0);
+    pan_start[(size_t)NPAN] = M;
+    BHS_HIP(hipMalloc(&im->d_prow_ptr, sizeof(int) * ((size_t)M + 1)));
+    BHS_HIP(hipMalloc(&d_owner, sizeof(int) * (size_t)std::max(M, 1)));
+    BHS_HIP(hipMalloc(&im->d_pcol, sizeof(int) * (size_t)im->nnz_pad));
+    BHS_HIP(hipMalloc(&im->d_pval, sizeof(T) * (size_t)im->nnz_pad));
+    BHS_HIP(hipMalloc(&im->d_ystack, sizeof(T) * (size_t)std::max(M, 1)));
+    BHS_HIP(hipMemsetAsync(im->d_pcol, 0, sizeof(int) * (size_t)im->nnz_pad, im->stream));
+    BHS_HIP(hipMemsetAsync(im->d_pval, 0, sizeof(T) * (size_t)im->nnz_pad, im->stream));
+    hipLaunchKernelGGL(bhs::panel_index_kernel<NPAN>, dim3((unsigned)((PM + threads - 1) / threads)),
+                       dim3(threads), 0, im->stream, (const int *)d_cnt, (const int *)d_rows,
+                       (const int *)d_nzs, m, im->d_prow_ptr, d_owner, im->d_pbase);
+    hipLaunchKernelGGL(bhs::fill_kernel<int>, dim3(1), dim3(64), 0, im->stream, im->d_prow_ptr + M,
+                       1LL, nnz);
+    hipLaunchKernelGGL((bhs::panel_scatter_kernel<T, NPAN>), dim3(row_blocks), dim3(threads), 0,
+                       im->stream,
+                       (const int *)im->d_row_ptr, (const int *)im->d_col, (const T *)im->d_val, m,
+                       pw, (const int *)d_cnt, (const int *)d_nzs, im->d_pcol, im->d_pval);
+    BHS_HIP(hipGetLastError());
+    std::vector<int> prp((size_t)M + 1), prow_of((size_t)M);
+    BHS_HIP(hipMemcpyAsync(prp.data(), im->d_prow_ptr, sizeof(int) * ((size_t)M + 1),
+                           hipMemcpyDeviceToHost, im->stream));
+    if (M > 0)
+        BHS_HIP(hipMemcpyAsync(prow_of.data(), d_owner, sizeof(int) * (size_t)M,
+                               hipMemcpyDeviceToHost, im->stream));
+    BHS_HIP(hipStreamSynchronize(im->stream));
+    im->h_prow_ptr = std::move(prp);
+    im->h_prow_of = std::move(prow_of);
+    im->h_pan_start = std::move(pan_start);
+    im->pan_n = NPAN;
+    im->pan_rows = M;
+    return BHSPARSE_SUCCESS;
+}
+
 template <typename T>
 int build_panels(Impl<T> *im, int npan)
 {
@@ -332,94 +427,13 @@ int build_panels(Impl<T> *im, int npan)
     im->h_prow_of = {};
     im->h_pan_start = {};
     im->pan_n = im->pan_rows = 0;
-    const int m = im->m, n = im->n, nnz = im->nnz;
-    if ((long long)npan * m > 0x7fffffffLL)
+    if ((long long)npan * im->m > 0x7fffffffLL)
         return set_error(BHSPARSE_INVALID_ARGUMENT, "too many rows for column panels");
-    std::vector<int> rp((size_t)m + 1), col((size_t)nnz);
-    std::vector<T> val((size_t)nnz);
-    BHS_HIP(hipMemcpyAsync(rp.data(), im->d_row_ptr, sizeof(int) * ((size_t)m + 1),
-                           hipMemcpyDeviceToHost, im->stream));
-    if (nnz > 0) {
-        BHS_HIP(hipMemcpyAsync(col.data(), im->d_col, sizeof(int) * (size_t)nnz,
-                               hipMemcpyDeviceToHost, im->stream));
-        BHS_HIP(hipMemcpyAsync(val.data(), im->d_val, sizeof(T) * (size_t)nnz,
-                               hipMemcpyDeviceToHost, im->stream));
+    switch (npan) {
+    case 2: return build_panels_n<T, 2>(im);
+    case 4: return build_panels_n<T, 4>(im);
+    default: return build_panels_n<T, 8>(im);
     }
-    BHS_HIP(hipStreamSynchronize(im->stream));
-    const int pw = (n + npan - 1) / npan;
-    const size_t PM = (size_t)npan * (size_t)m;
-    std::vector<int> cnt(PM, 0);   // entries of row i in panel p, at p*m+i
-    for (int i = 0; i < m; ++i)
-        for (int j = rp[i]; j < rp[i + 1]; ++j) ++cnt[(size_t)(col[j] / pw) * m + i];
-    std::vector<int> idx(PM), prp(1, 0), prow_of, pan_start((size_t)npan + 1, 0);
-    prp.reserve(PM + 1);
-    for (size_t q = 0; q < PM; ++q) {
-        if (q % (size_t)m == 0) pan_start[q / (size_t)m] = (int)prp.size() - 1;
-        if (cnt[q] > 0) {
-            idx[q] = (int)prp.size() - 1;
-            prp.push_back(prp.back() + cnt[q]);
-            prow_of.push_back((int)(q % (size_t)m));
-        } else {
-            idx[q] = -1;
-        }
-    }
-    const int M = (int)prp.size() - 1;
-    pan_start[(size_t)npan] = M;
-    // combine metadata: per-row panel bits, per-(64-row group, panel) base
-    const long long G = ((long long)m + 63) / 64;
-    std::vector<unsigned char> mask((size_t)m, 0);
-    std::vector<int> base((size_t)(G * npan));
-    for (int p = 0; p < npan; ++p) {
-        int next = -1;   // first stacked row of panel p at or after the group
-        for (long long g = G - 1; g >= 0; --g) {
-            for (long long i = std::min<long long>(m, (g + 1) * 64) - 1; i >= g * 64; --i) {
-                const int k = idx[(size_t)p * m + (size_t)i];
-                if (k >= 0) {
-                    next = k;
-                    mask[(size_t)i] |= (unsigned char)(1u << p);
-                }
-            }
-            base[(size_t)(g * npan + p)] = next;
-        }
-    }
-    // a group with no row in panel p never reads its base; give it any value
-    for (auto &b : base)
-        if (b < 0) b = 0;
-    std::vector<int> pos(prp.begin(), prp.end() - 1), pcol((size_t)nnz);
-    std::vector<T> pval((size_t)nnz);
-    for (int i = 0; i < m; ++i)
-        for (int j = rp[i]; j < rp[i + 1]; ++j) {
-            const int dst = pos[(size_t)idx[(size_t)(col[j] / pw) * m + i]]++;
-            pcol[(size_t)dst] = col[j];
-            pval[(size_t)dst] = val[j];
-        }
-    BHS_HIP(hipMalloc(&im->d_prow_ptr, sizeof(int) * ((size_t)M + 1)));
-    BHS_HIP(hipMalloc(&im->d_pcol, sizeof(int) * (size_t)im->nnz_pad));
-    BHS_HIP(hipMalloc(&im->d_pval, sizeof(T) * (size_t)im->nnz_pad));
-    BHS_HIP(hipMalloc(&im->d_ystack, sizeof(T) * (size_t)std::max(M, 1)));
-    BHS_HIP(hipMalloc(&im->d_pmask, std::max<size_t>((size_t)m, 1)));
-    BHS_HIP(hipMalloc(&im->d_pbase, sizeof(int) * std::max<size_t>(base.size(), 1)));
-    BHS_HIP(hipMemcpyAsync(im->d_prow_ptr, prp.data(), sizeof(int) * ((size_t)M + 1),
-                           hipMemcpyHostToDevice, im->stream));
-    BHS_HIP(hipMemcpyAsync(im->d_pmask, mask.data(), (size_t)m, hipMemcpyHostToDevice,
-                           im->stream));
-    BHS_HIP(hipMemcpyAsync(im->d_pbase, base.data(), sizeof(int) * base.size(),
-                           hipMemcpyHostToDevice, im->stream));
-    BHS_HIP(hipMemsetAsync(im->d_pcol, 0, sizeof(int) * (size_t)im->nnz_pad, im->stream));
-    BHS_HIP(hipMemsetAsync(im->d_pval, 0, sizeof(T) * (size_t)im->nnz_pad, im->stream));
-    if (nnz > 0) {
-        BHS_HIP(hipMemcpyAsync(im->d_pcol, pcol.data(), sizeof(int) * (size_t)nnz,
-                               hipMemcpyHostToDevice, im->stream));
-        BHS_HIP(hipMemcpyAsync(im->d_pval, pval.data(), sizeof(T) * (size_t)nnz,
-                               hipMemcpyHostToDevice, im->stream));
-    }
-    BHS_HIP(hipStreamSynchronize(im->stream));
-    im->h_prow_ptr = std::move(prp);
-    im->h_prow_of = std::move(prow_of);
-    im->h_pan_start = std::move(pan_start);
-    im->pan_n = npan;
-    im->pan_rows = M;
-    return BHSPARSE_SUCCESS;
 }
 
 // Chain records of the panel combine for waves of PW stacked nonzeros.  Wave

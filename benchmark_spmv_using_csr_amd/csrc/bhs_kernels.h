@@ -999,6 +999,102 @@ __global__ __launch_bounds__(256) void combine_panels_kernel(const T *__restrict
     }
 }
 
+// ---------------------------------------------------------------------------
+// Device build of the column-panel copy (count -> scan -> scatter).  Panel of
+// column c: c / pw.  The stacked matrix has one row per nonempty (panel, row)
+// pair, panel-major; within a stacked row the entries keep their order in A.
+// ---------------------------------------------------------------------------
+// cnt[p*m + i] = entries of row i in panel p; mask[i] bit p = (cnt > 0).  One
+// wave per row, 64 entries at a time (a row may hold 10^5 entries).
+template <int NPAN>
+__global__ __launch_bounds__(256) void panel_count_kernel(const int *__restrict__ row_ptr,
+                                                         const int *__restrict__ col, int m,
+                                                         int pw, int *__restrict__ cnt,
+                                                         unsigned char *__restrict__ mask)
+{
+    const int lane = lane_id();
+    const long long i = __builtin_amdgcn_readfirstlane(
+        (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    if (i >= m) return;
+    const int b = row_ptr[i], e = row_ptr[i + 1];
+    int c[NPAN];
+#pragma unroll
+    for (int p = 0; p < NPAN; ++p) c[p] = 0;
+    for (int j = b; j < e; j += kWave) {
+        const int q = j + lane < e ? col[j + lane] / pw : -1;
+#pragma unroll
+        for (int p = 0; p < NPAN; ++p) c[p] += __popcll(ballot(q == p));
+    }
+    if (lane == 0) {
+        unsigned mk = 0;
+#pragma unroll
+        for (int p = 0; p < NPAN; ++p) {
+            cnt[(long long)p * m + i] = c[p];
+            mk |= (c[p] > 0 ? 1u : 0u) << p;
+        }
+        mask[i] = (unsigned char)mk;
+    }
+}
+
+// With flag[q] = (cnt[q] > 0) and rows[q] / nzs[q] the exclusive sums of the
+// flags / counts over q = p*m + i: the stacked row of a nonempty pair is
+// rows[q]; its start is nzs[q]; base[g*NPAN + p] = rows[p*m + 64g] is the first
+// stacked row of panel p at or after row 64g (read only for groups that have
+// rows in panel p); owner[R] = the row of A that stacked row R belongs to.
+template <int NPAN>
+__global__ __launch_bounds__(256) void panel_index_kernel(const int *__restrict__ cnt,
+                                                         const int *__restrict__ rows,
+                                                         const int *__restrict__ nzs, int m,
+                                                         int *__restrict__ prow_ptr,
+                                                         int *__restrict__ owner,
+                                                         int *__restrict__ base)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)NPAN * m) return;
+    const int p = (int)(q / m), i = (int)(q % m);
+    if (cnt[q] > 0) {
+        prow_ptr[rows[q]] = nzs[q];
+        owner[rows[q]] = i;
+    }
+    if ((i & 63) == 0) base[(long long)(i >> 6) * NPAN + p] = rows[q];
+}
+
+// Entries of row i to their stacked rows, in order: one wave per row, 64
+// entries at a time; an entry's place is its panel's running offset plus the
+// number of lower lanes in the same panel (ballot + mbcnt).
+template <typename T, int NPAN>
+__global__ __launch_bounds__(256) void panel_scatter_kernel(
+    const int *__restrict__ row_ptr, const int *__restrict__ col, const T *__restrict__ val,
+    int m, int pw, const int *__restrict__ cnt, const int *__restrict__ nzs,
+    int *__restrict__ pcol, T *__restrict__ pval)
+{
+    const int lane = lane_id();
+    const long long i = __builtin_amdgcn_readfirstlane(
+        (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    if (i >= m) return;
+    const int b = row_ptr[i], e = row_ptr[i + 1];
+    int off[NPAN];
+#pragma unroll
+    for (int p = 0; p < NPAN; ++p) off[p] = nzs[(long long)p * m + i];
+    for (int j = b; j < e; j += kWave) {
+        const bool in = j + lane < e;
+        const int c = in ? col[j + lane] : 0;
+        const T v = in ? val[j + lane] : T(0);
+        const int q = in ? c / pw : -1;
+        int d = 0;
+#pragma unroll
+        for (int p = 0; p < NPAN; ++p) {
+            const uint64_t bits = ballot(q == p);
+            if (q == p) d = off[p] + popc_below(bits);
+            off[p] += __popcll(bits);
+        }
+        if (in) {
+            pcol[d] = c;
+            pval[d] = v;
+        }
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void fill_kernel(T *__restrict__ p, long long n, T v)
 {

@@ -61,6 +61,9 @@ def main(argv=None):
         print(f"batched {len(mats)} matrices, {sum(ks)} right-hand sides, schedule {a.batched}: "
               f"median {np.median(ms):.4f} ms over {a.reps} graph replays", flush=True)
         b.close()
+        if a.copy_mb > 0:                  # the counters' calibration, in the same process
+            cm = stream_copy(a.copy_mb << 20, reps=a.reps)
+            print(f"copy {a.copy_mb} MiB: median {np.median(cm) * 1e3:.2f} us", flush=True)
         return 0
     dtype = np.float64 if a.dtype == "f64" else np.float32
     A, x = workload(a.workload)

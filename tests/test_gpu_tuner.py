@@ -131,11 +131,13 @@ def test_autotune_installs_fastest_and_stays_correct():
 
 
 def test_learned_tuner_live_on_held_out_matrices():
-    """Live on the GPU: the committed network's pick vs the exhaustive best and
-    the fixed default tile, on held-out matrices of the tuner set."""
+    """Live on the GPU: the committed network's pick (and its timed shortlist
+    of three) vs the exhaustive best and the fixed default tile, on every
+    tenth held-out matrix of the tuner set.  The bound is the target's: the
+    tuned time within 5% of the exhaustive best (geometric mean)."""
     net = tuner.make_net(tuner.load_net())
-    to_best, vs_default = [], []
-    for seed in range(1500, 2000, 23):
+    to_best, vs_default, short_best = [], [], []
+    for seed in range(1500, 2000, 10):
         A, dtype = tuner.dataset_matrix(seed)
         if A.nnz > 3e7:
             continue
@@ -145,13 +147,17 @@ def test_learned_tuner_live_on_held_out_matrices():
         _lib.check(op.prepare_mem(A.m, A.n, A.nnz, A.row_ptr, A.col, A.val.astype(dtype),
                                   matgen.vector(seed, A.n).astype(dtype), y), "prepare_mem")
         chosen = op.apply_tuner(net, tuner.CLASSES)
+        short = op.apply_tuner(net, tuner.CLASSES, topk=3, reps=5, flush_bytes=1 << 30)
         us, best = op.autotune(tuner.CLASSES, reps=7, flush_bytes=1 << 30)
         to_best.append(us[chosen] / us[best])
+        short_best.append(us[short] / us[best])
         vs_default.append(us[tuner.DEFAULT_CLASS] / us[chosen])
         op.close()
     gm = lambda a: float(np.exp(np.mean(np.log(a))))
     print(f"[gpu] live held-out: n={len(to_best)} tuned/best {gm(to_best):.4f} "
+          f"(p90 {np.percentile(to_best, 90):.4f}) shortlist/best {gm(short_best):.4f} "
           f"default/tuned {gm(vs_default):.4f}", flush=True)
-    assert len(to_best) >= 15
-    assert gm(to_best) <= 1.08
+    assert len(to_best) >= 40
+    assert gm(to_best) <= 1.05
+    assert gm(short_best) <= 1.05
     assert gm(vs_default) >= 1.2
