@@ -145,3 +145,55 @@ def test_noalloc_rejects_what_it_cannot_hold():
         rc = L.bhs_d_spmv_noalloc(1, 1, 1, ip, ip, C.c_void_p(p.value + 8), p, p, 1.0, None)
         assert rc == -3
         assert L.bhs_d_spmv_noalloc(-1, 1, 1, ip, ip, p, p, p, 1.0, None) == -3
+
+
+def test_noalloc_under_graph_capture():
+    """The callers it is for: captured into a hipGraph on the caller's stream and
+    replayed, with x changed and eager work done between replays; every replay
+    gives the eager call's y for the current x, bit for bit."""
+    H = hip()
+    for f, args in (("hipStreamCreate", [C.POINTER(C.c_void_p)]),
+                    ("hipStreamBeginCapture", [C.c_void_p, C.c_int]),
+                    ("hipStreamEndCapture", [C.c_void_p, C.POINTER(C.c_void_p)]),
+                    ("hipGraphInstantiate", [C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p,
+                                             C.c_void_p, C.c_size_t]),
+                    ("hipGraphLaunch", [C.c_void_p, C.c_void_p]),
+                    ("hipStreamSynchronize", [C.c_void_p]),
+                    ("hipGraphExecDestroy", [C.c_void_p]), ("hipGraphDestroy", [C.c_void_p]),
+                    ("hipStreamDestroy", [C.c_void_p])):
+        getattr(H, f).argtypes = args
+        getattr(H, f).restype = C.c_int
+    c = power_law_case(31, 50_000, 40_000, 400_001)
+    rng = np.random.default_rng(4)
+    xs = [c.x] + [rng.uniform(-1, 1, c.n) for _ in range(3)]
+    say(f"noalloc graph capture {c.name}")
+    y0 = np.zeros(c.m)
+    with DeviceArrays(np.asarray(c.row_ptr, np.int32), np.asarray(c.col, np.int32), c.val,
+                      c.x, y0) as d:
+        st = C.c_void_p()
+        assert H.hipStreamCreate(C.byref(st)) == 0
+        # eager results for every x, on the same stream
+        eager = []
+        for xv in xs:
+            assert H.hipMemcpy(d.ptrs[3], np.ascontiguousarray(xv).ctypes.data_as(C.c_void_p),
+                               xv.nbytes, _H2D) == 0
+            _lib.check(call_noalloc(np.float64, c.m, c.n, c.nnz, d.ptrs, 1.0, st), "noalloc")
+            assert H.hipStreamSynchronize(st) == 0
+            eager.append(d.get(4, y0).copy())
+        graph, gexec = C.c_void_p(), C.c_void_p()
+        assert H.hipStreamBeginCapture(st, 0) == 0             # hipStreamCaptureModeGlobal
+        rc = call_noalloc(np.float64, c.m, c.n, c.nnz, d.ptrs, 1.0, st)
+        assert H.hipStreamEndCapture(st, C.byref(graph)) == 0
+        _lib.check(rc, "noalloc under capture")
+        assert H.hipGraphInstantiate(C.byref(gexec), graph, None, None, 0) == 0
+        for k in (1, 2, 3, 0):
+            xv = xs[k]
+            assert H.hipMemcpy(d.ptrs[3], np.ascontiguousarray(xv).ctypes.data_as(C.c_void_p),
+                               xv.nbytes, _H2D) == 0
+            # eager work between replays, on the device's other arrays
+            assert H.hipMemcpy(d.ptrs[4], y0.ctypes.data_as(C.c_void_p), y0.nbytes, _H2D) == 0
+            assert H.hipGraphLaunch(gexec, st) == 0
+            assert H.hipStreamSynchronize(st) == 0
+            assert np.array_equal(d.get(4, y0), eager[k]), k
+        assert H.hipGraphExecDestroy(gexec) == 0 and H.hipGraphDestroy(graph) == 0
+        assert H.hipStreamDestroy(st) == 0
