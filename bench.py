@@ -40,10 +40,11 @@ def eff_bytes(m, n, nnz, vsize, k=1):
     return nnz * (vsize + 4) + 4 * (m + 1) + k * (n + m) * vsize
 
 
-def load_traffic(name, cfg):
+def load_traffic(name, cfg, key="config"):
     """HBM bytes per launch of the main kernel from the committed rocprofv3
     --pmc summary for this workload and tile (FETCH_SIZE + WRITE_SIZE, scaled by
-    the copy-kernel calibration recorded with it); None if absent."""
+    the copy-kernel calibration recorded with it); None if absent.  key: the
+    entry field that names the tile ("config", or "schedule" for the batch)."""
     p = os.path.join(ROOT, "profiles", name)
     if not name or not os.path.exists(p):
         return None
@@ -51,7 +52,7 @@ def load_traffic(name, cfg):
         with open(p) as f:
             d = json.load(f)
         for e in d.get("entries", [d]):
-            if tuple(e.get("config", ())) == tuple(cfg):
+            if tuple(e.get(key, ())) == tuple(cfg):
                 return float(e["hbm_bytes_per_launch"])
         return None
     except Exception:
@@ -180,7 +181,8 @@ def run_batched():
         "batched_effective_gbps": round(ach, 1),
         "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBPS, 4),
-                     "traffic": None,
+                     "traffic": load_traffic("batched_256mat_f64_traffic.json", sched,
+                                             key="schedule"),
                      "definition": "effective bytes of the batch (A once, x and y per "
                                    "right-hand side) / median graph replay (queue reset, "
                                    "batched_kernel, calibration)"},
@@ -189,6 +191,62 @@ def run_batched():
                      "order": "matrix" if sched[4] == 0 else "most work first"},
         "batched_equals_plain_bitwise": bool(identical), "setup_s": round(setup, 2),
     }
+
+
+OTHER_WORKLOADS = (("power-law", "f32"), ("laplace", "f64"), ("block3", "f64"),
+                   ("uniform", "f64"))
+
+
+def workload(name):
+    """The synthetic matrices and their seeded x vectors."""
+    if name == "power-law":
+        return matgen.power_law_1m(), matgen.vector(102, 1 << 20)
+    if name == "uniform":
+        return matgen.uniform_65536(), matgen.vector(101, 65536)
+    if name == "laplace":
+        A = matgen.laplacian_160()
+        return A, matgen.vector(103, A.n)
+    A = matgen.block3_2m()
+    return A, matgen.vector(104, A.n)
+
+
+def run_others(flush_bytes, vendor_cols, reps=30):
+    """The other benchmark matrices (and fp32 on the power-law one), each with
+    the learned tuner's shortlisted tile, timed cold like the headline, beside
+    rocSPARSE's best algorithm timed the same way."""
+    from benchmark_spmv_using_csr_amd import tuner
+
+    net = tuner.make_net(tuner.load_net())
+    rows = []
+    for wl, dt in OTHER_WORKLOADS:
+        dtype = np.float64 if dt == "f64" else np.float32
+        A, x = workload(wl)
+        op = bhsparse_spmv(dtype)
+        _lib.check(op.init_platform(0), "init_platform")
+        y = np.zeros(A.m, dtype)
+        _lib.check(op.prepare_mem(A.m, A.n, A.nnz, A.row_ptr, A.col, A.val.astype(dtype),
+                                  x.astype(dtype), y), "prepare_mem")
+        cfg = tuner.CLASSES[op.apply_tuner(net, tuner.CLASSES, topk=TUNER_TOPK, reps=7,
+                                           flush_bytes=flush_bytes)]
+        tot, main_ms = op.bench(1.0, warmup=5, reps=reps, flush_bytes=flush_bytes)
+        _lib.check(op.get_y(), "get_y")
+        op.close()
+        b = eff_bytes(A.m, A.n, A.nnz, np.dtype(dtype).itemsize)
+        med, km = median(tot), float(np.mean(main_ms))
+        row = {"workload": A.name, "dtype": dt, "m": A.m, "nnz": A.nnz,
+               "tile": tuner.class_name(cfg), "ms_per_step": round(med, 5),
+               "gbps": round(b / (med * 1e-3) / 1e9, 1),
+               "kernel_mean_us": round(km * 1e3, 3),
+               "roofline_frac": round(b / (km * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+               "whole_call_frac": round(b / (med * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
+        if vendor_cols:
+            v = run_vendor(A, x, dtype, flush_bytes, y, reps=10)
+            row["vendor_best"] = {"alg": v["best_alg"], "cold_us": v["best_cold_us"],
+                                  "max_abs_diff_vs_bhs_over_max_y":
+                                      v["algs"][v["best_alg"]]["max_abs_diff_vs_bhs_over_max_y"]}
+            row["bhs_speedup_vs_vendor_best"] = round(v["best_cold_us"] / (med * 1e3), 3)
+        rows.append(row)
+    return rows
 
 
 def run_vendor(A, x, dtype, flush_bytes, y_bhs, reps=20):
@@ -272,6 +330,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-vendor", action="store_true")
+    ap.add_argument("--no-others", action="store_true",
+                    help="skip the other matrices (fp32 power-law, Laplacian, block3, uniform)")
     ap.add_argument("--workload", choices=["power-law", "uniform", "laplace", "block3"],
                     default="power-law")
     ap.add_argument("--mtx", default="",
@@ -295,16 +355,8 @@ def main():
         A = matgen.read_mtx(a.mtx)
         A.name = f"Matrix Market file {os.path.basename(a.mtx)} m={A.m} n={A.n}"
         x = matgen.vector(105, A.n)
-    elif a.workload == "power-law":
-        A, x = matgen.power_law_1m(), matgen.vector(102, 1 << 20)
-    elif a.workload == "uniform":
-        A, x = matgen.uniform_65536(), matgen.vector(101, 65536)
-    elif a.workload == "laplace":
-        A = matgen.laplacian_160()
-        x = matgen.vector(103, A.n)
     else:
-        A = matgen.block3_2m()
-        x = matgen.vector(104, A.n)
+        A, x = workload(a.workload)
 
     managed = a.memory != "device"
     r = run_gpu(A, x, dtype, cfg, a.warmup, a.steps, a.flush_mb << 20,
@@ -441,6 +493,8 @@ def main():
         out["vendor"] = v
     if not a.no_batched:
         out["batched"] = run_batched()
+    if not a.no_others and not a.mtx and a.memory == "device":
+        out["other_workloads"] = run_others(a.flush_mb << 20, not a.no_vendor)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A, x, dtype, r["y"])
     print(json.dumps(out), flush=True)

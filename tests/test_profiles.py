@@ -60,3 +60,19 @@ def test_bench_reads_the_entries():
     assert bench.load_traffic("segsum_powerlaw_f64_traffic.json", (0, 16, 8, 99, 0)) is None
     assert bench.load_traffic("", (2, 8, 1, 8, 4)) is None
     assert np.isfinite(bench.HBM_PEAK_GBPS)
+
+
+def test_batched_traffic_entries_are_consistent():
+    from benchmark_spmv_using_csr_amd import batched
+    with open(os.path.join(ROOT, "profiles", "batched_256mat_f64_traffic.json")) as f:
+        d = json.load(f)
+    assert d["entries"]
+    for e in d["entries"]:
+        assert tuple(e["schedule"]) in batched.SCHEDULES, e["schedule"]
+        cal = e["copy_calibration"]
+        rf = cal["bytes_read_kib"] / cal["fetch_size_kib"]
+        wf = cal["bytes_written_kib"] / cal["write_size_kib"]
+        hb = (e["fetch_size_kib"] * rf + e["write_size_kib"] * wf) * 1024
+        assert abs(hb - e["hbm_bytes_per_launch"]) <= 2e-6 * hb
+        assert e["hbm_bytes_per_launch"] > e["algorithmic_bytes_per_launch"]
+        assert os.path.exists(os.path.join(ROOT, e["source"])), e["source"]
