@@ -188,7 +188,8 @@ def run_batched():
                                    "batched_kernel, calibration)"},
         "schedule": {"items_per_lane": sched[0], "tiles_per_wave": sched[1],
                      "waves_per_block": sched[2], "grid_blocks": sched[3],
-                     "order": "matrix" if sched[4] == 0 else "most work first"},
+                     "order": "matrix" if sched[4] == 0 else "most work first",
+                     "x_stage": sched[5]},
         "batched_equals_plain_bitwise": bool(identical), "setup_s": round(setup, 2),
     }
 

@@ -51,11 +51,12 @@ class SpmvStats(C.Structure):
 
 class BatchSchedule(C.Structure):
     _fields_ = [("items_per_lane", C.c_int), ("tiles_per_wave", C.c_int),
-                ("waves_per_block", C.c_int), ("grid_blocks", C.c_int), ("order", C.c_int)]
+                ("waves_per_block", C.c_int), ("grid_blocks", C.c_int), ("order", C.c_int),
+                ("x_stage", C.c_int)]
 
     def as_tuple(self):
         return (self.items_per_lane, self.tiles_per_wave, self.waves_per_block,
-                self.grid_blocks, self.order)
+                self.grid_blocks, self.order, self.x_stage)
 
 
 class Csr(C.Structure):

@@ -56,8 +56,9 @@ class SpmvBatch:
         check(self._fn("batch_set_x")(self.h, _ptrs(self._x, None)), "batch_set_x")
 
     def set_schedule(self, items_per_lane=8, tiles_per_wave=4, waves_per_block=4, grid_blocks=0,
-                     order=1):
-        s = BatchSchedule(items_per_lane, tiles_per_wave, waves_per_block, grid_blocks, order)
+                     order=1, x_stage=0):
+        s = BatchSchedule(items_per_lane, tiles_per_wave, waves_per_block, grid_blocks, order,
+                          x_stage)
         check(self._fn("batch_set_schedule")(self.h, C.byref(s)), "batch_set_schedule")
 
     def schedule(self):
@@ -103,9 +104,12 @@ class SpmvBatch:
             pass
 
 
-# Candidate schedules for the batch autotuner: (S, T, waves/block, grid, order).
-SCHEDULES = [(S, T, wpb, 0, 1) for S in (4, 8) for T in (1, 4, 16) for wpb in (1, 4)] + \
-            [(4, T, wpb, 0, 1) for T in (32, 64) for wpb in (1, 4)] + [(8, 4, 4, 0, 0)]
+# Candidate schedules for the batch autotuner: (S, T, waves/block, grid, order,
+# x_stage).  The x-staged ones run on fp64 batches only (the autotuner skips
+# them elsewhere).
+SCHEDULES = [(S, T, wpb, 0, 1, 0) for S in (4, 8) for T in (1, 4, 16) for wpb in (1, 4)] + \
+            [(4, T, wpb, 0, 1, 0) for T in (32, 64) for wpb in (1, 4)] + [(8, 4, 4, 0, 0, 0)] + \
+            [(4, T, 1, 0, 1, xs) for xs in (2, 4) for T in (8, 16, 32)]
 
 
 def batch_set(seed=5, n_mats=256, m_lo=1e3, m_hi=2e5, dtype=np.float64):

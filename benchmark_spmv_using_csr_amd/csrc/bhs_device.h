@@ -238,6 +238,23 @@ __device__ __forceinline__ u32x2 load_nt8(const void *p)
     return __builtin_nontemporal_load(reinterpret_cast<const u32x2 *>(p));
 }
 
+// 16 bytes from global memory straight into LDS (global_load_lds_dwordx4, no
+// registers): the lanes of one call write one contiguous 1 KB block, lane l at
+// lds_block + 16*l.  lds_block is wave-uniform.  Completion is counted by
+// vmcnt; lds_dma_wait() makes the block readable by every lane of the wave.
+__device__ __forceinline__ void load_lds16(const void *src, void *lds_block)
+{
+    __builtin_amdgcn_global_load_lds(
+        (__attribute__((address_space(1))) const void *)src,
+        (__attribute__((address_space(3))) void *)lds_block, 16, 0, 0);
+}
+
+__device__ __forceinline__ void lds_dma_wait()
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    wave_sync();
+}
+
 // v if keep, else +0, without a branch (bitwise select keeps loads feeding v
 // unconditional).
 __device__ __forceinline__ double keep_or_zero(double v, bool keep)

@@ -314,6 +314,90 @@ struct RhsPair {
     }
 };
 
+// Right-hand sides in pairs with the tile's x values staged in LDS (fp64,
+// S = 4, k even): an item's k values are k*8 contiguous bytes of X (one
+// 128-byte line at k = 16), cut into C = k/2 16-byte chunks, one pair each.
+// Staged XCH chunks at a time (a pass), the chunks of an item come from one
+// load instruction, on neighbouring lanes, so each line is one request to
+// the cache per pass instead of one per pair; the pairs are then read from
+// LDS.  (Loading the next pass into a second buffer while the current one is
+// reduced measured slower: the extra LDS costs more resident waves than the
+// overlap gains.)
+//
+// Layout of a pass (Cp = min(C, XCH) rounded up to a power of two, the same
+// for every pass of a tile; M = 16/Cp): load instruction i in [0, 4*Cp)
+// writes LDS bytes [1024 i, 1024 i + 1024), lane l at 16 l.  Lane l of
+// instruction i loads chunk c of item t where, with s = l / Cp, q = l % Cp,
+// r = s % M, w = s / M, e = 4 i + w, v = e / 2:
+//     t = 2 (v M + r) + e % 2,    c = (q - v) mod Cp.
+// A pair read has lane L take item t = 128 j + 2 L + g (fixed j, g) at one
+// chunk c: lanes L = 0..15 then land in the 16 distinct 16-byte bank groups
+// (r = L mod M, q = (c + L / M) mod Cp), so the reads are conflict-free.
+template <int XCH>
+struct XStage {
+    unsigned char *buf;                            // 1024 * 4 * XCH bytes
+    int *s_col;                                    // the tile's column indices
+    int lc = 0, lm = 4, q = 0, w = 0, tb = 0;     // the tile's pass geometry, this lane
+
+    // The tile's columns (lane L holds items 128j+2L+g) and the pass geometry:
+    // the item this lane loads in instruction i is tb + i * 4M.
+    __device__ __forceinline__ void begin_tile(const Tile<double, 4> &t, int lane, int C)
+    {
+        wave_sync();                               // earlier reads of s_col are done
+        s_col[2 * lane] = t.c[0];
+        s_col[2 * lane + 1] = t.c[1];
+        s_col[128 + 2 * lane] = t.c[2];
+        s_col[128 + 2 * lane + 1] = t.c[3];
+        const int cw = C < XCH ? C : XCH;
+        lc = cw <= 1 ? 0 : cw <= 2 ? 1 : cw <= 4 ? 2 : 3;
+        lm = 4 - lc;
+        const int s = lane >> lc;
+        q = lane & ((1 << lc) - 1);
+        w = s >> lm;
+        tb = 2 * (((w >> 1) << lm) + (s & ((1 << lm) - 1))) + (w & 1);
+        wave_sync();
+    }
+    // Issues the loads of chunks [c0, c0 + cg) of every item (pair() waits
+    // for them when it first reads the pass).
+    __device__ __forceinline__ void stage(const double *__restrict__ x, int ks, int c0, int cg)
+    {
+        const int Cm = (1 << lc) - 1;
+        wave_sync();                               // earlier reads of buf are done
+        const int n = 4 << lc;                     // a multiple of 4
+#pragma unroll 4
+        for (int i = 0; i < n; ++i) {
+            const int c = (q - 2 * i - (w >> 1)) & Cm;
+            if (c < cg) {
+                const int cl = s_col[tb + (i << (lm + 2))];
+                load_lds16(x + (long long)cl * ks + 2 * (c0 + c), buf + 1024 * i);
+            }
+        }
+    }
+    // The x values of pair rr (k even) for this lane's S = 4 items.  The
+    // tile's first pass is issued at the top of the tile (so its loads are in
+    // flight during the tile's flag and row work); later passes when their
+    // first pair is reached.
+    __device__ __forceinline__ void pair(const double *__restrict__ x, int ks, int k, int rr,
+                                         int lane, double2v (&d)[4])
+    {
+        const int C = k >> 1, c0 = rr >> 1, cl = c0 % XCH;
+        if (cl == 0) {
+            if (c0 > 0) stage(x, ks, c0, min(C - c0, XCH));
+            lds_dma_wait();
+        }
+        const int Cm = (1 << lc) - 1, Mm = (1 << lm) - 1;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const int u = (kk >> 1) * 64 + lane;   // item t = 2u + (kk & 1)
+            const int r = u & Mm, v = u >> lm;
+            const int e = 2 * v + (kk & 1);
+            const int s = r + ((e & 3) << lm);
+            const int l = (s << lc) + ((cl + v) & Cm);
+            d[kk] = *reinterpret_cast<const double2v *>(buf + 1024 * (e >> 2) + 16 * l);
+        }
+    }
+};
+
 template <int R>
 struct RhsTag {
     static constexpr int value = R;
@@ -321,10 +405,15 @@ struct RhsTag {
 
 // BOUNDED: col/val hold exactly nnz items (no padding), so a tile reaching
 // past nnz is loaded with checked indices (a wave-uniform branch).
-template <typename T, int S, int KMAX, bool STRIDED, bool STATS, int CP, bool BOUNDED = false>
+// XCH > 0 (fp64, S = 4, KMAX > 1): right-hand-side pairs take their x values
+// from the tile staged in LDS, XCH chunks per item at a time (XStage), at
+// s_stage: 1 KB of column indices, then the staging buffer.
+template <typename T, int S, int KMAX, bool STRIDED, bool STATS, int CP, bool BOUNDED = false,
+          int XCH = 0>
 __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, const int lane,
                                             unsigned *s_head, T *s_buf, T *s_carry, const T alpha,
-                                            unsigned long long *stats)
+                                            unsigned long long *stats,
+                                            unsigned char *s_stage = nullptr)
 {
     constexpr int P = kWave * S;
     constexpr int G = Tile<T, S>::G;
@@ -335,6 +424,9 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
     // tile's pairs fit the registers (S * sizeof(T) <= 32 bytes per lane)
     constexpr bool kPairs = KMAX > 1 && S * sizeof(T) <= 32;
     const bool pairs = kPairs && (ks & 1) == 0 && (k & 1) == 0;
+    constexpr bool kStage = XCH > 0 && kPairs && std::is_same<T, double>::value && S == 4;
+    [[maybe_unused]] XStage<(XCH > 0 ? XCH : 1)> xs{kStage ? s_stage + 1024 : nullptr,
+                                                   reinterpret_cast<int *>(s_stage)};
     const int *__restrict__ row_ptr = mv.row_ptr;
     const int *__restrict__ tile_row = mv.tile_row;
     const T *__restrict__ x = mv.x;
@@ -404,7 +496,14 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
         const bool general = r0 != r1;
         RhsX<T, S, KMAX> xr;
         [[maybe_unused]] RhsPair<T, S> xp;
-        if constexpr (kPairs) {
+        if constexpr (kStage) {
+            if (pairs) {
+                xs.begin_tile(cur, lane, k >> 1);
+                xs.stage(x, ks, 0, min(k >> 1, XCH));   // waited for by the first pair
+            } else {
+                xr.first(x, cur, ks);
+            }
+        } else if constexpr (kPairs) {
             if (pairs) xp.first(x, cur, ks);
             else xr.first(x, cur, ks);
         } else {
@@ -442,7 +541,16 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                 carry_setv(rt, rr, cont ? carry_getv(rt, rr) + s : s);
             };
             bool done = false;
-            if constexpr (kPairs) {
+            if constexpr (kStage) {
+                if (pairs) {
+                    for (int rr = 0; rr < k; rr += 2) {
+                        double2v xv[4];
+                        xs.pair(x, ks, k, rr, lane, xv);
+                        fast_rhs(RhsTag<2>{}, rr, xv);
+                    }
+                    done = true;
+                }
+            } else if constexpr (kPairs) {
                 if (pairs) {
                     for (int rr = 0; rr < k; rr += 2) {
                         xp.issue(x, cur, ks, rr, k);
@@ -647,7 +755,16 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                     carry_setv(rt, rr, sb[K - 1]);
                 };
                 bool done = false;
-                if constexpr (kPairs) {
+                if constexpr (kStage) {
+                    if (pairs) {
+                        for (int rr = 0; rr < k; rr += 2) {
+                            double2v xv[4];
+                            xs.pair(x, ks, k, rr, lane, xv);
+                            general_rhs(RhsTag<2>{}, rr, xv);
+                        }
+                        done = true;
+                    }
+                } else if constexpr (kPairs) {
                     if (pairs) {
                         for (int rr = 0; rr < k; rr += 2) {
                             xp.issue(x, cur, ks, rr, k);
@@ -851,7 +968,7 @@ struct BatchMeta {
     int sync_off, m, nnz, NP, NW, lead_empty, k, pad;
 };
 
-template <typename T, int S, int KMAX>
+template <typename T, int S, int KMAX, int XCH>
 __global__ __launch_bounds__(512) void batched_kernel(
     const int *__restrict__ row_ptr, const int *__restrict__ col, const T *__restrict__ val,
     const T *__restrict__ x, T *__restrict__ y, const int *__restrict__ tile_row,
@@ -866,6 +983,9 @@ __global__ __launch_bounds__(512) void batched_kernel(
     unsigned *s_head = reinterpret_cast<unsigned *>(wave_lds);
     T *s_buf = reinterpret_cast<T *>(wave_lds + 256);   // room for 64*S pairs
     T *s_carry = s_buf + 2 * kWave * S;
+    // XCH > 0: column indices and the x staging buffer after the carry
+    unsigned char *s_stage =
+        wave_lds + (256 + 2 * kWave * S * sizeof(T) + KMAX * sizeof(T) + 15) / 16 * 16;
     for (;;) {
         int it = 0;
         if (lane == 0) it = atomicAdd(queue, 1);
@@ -878,8 +998,8 @@ __global__ __launch_bounds__(512) void batched_kernel(
                       sync_row + md.sync_off, sync_val + (long long)md.sync_off * KMAX,
                       md.m, md.nnz, md.NP, md.NW, md.lead_empty, md.k, md.k, TPW,
                       (int)md.y_off};
-        segsum_wave<T, S, KMAX, true, false, 2>(mv, iw.y, lane, s_head, s_buf, s_carry, alpha,
-                                             nullptr);
+        segsum_wave<T, S, KMAX, true, false, 2, false, XCH>(mv, iw.y, lane, s_head, s_buf,
+                                                           s_carry, alpha, nullptr, s_stage);
         if (lane == 0) sync_k[md.sync_off + iw.y] = md.k;
         wave_sync();                              // LDS is reused by the next item
     }

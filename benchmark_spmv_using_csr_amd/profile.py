@@ -48,8 +48,9 @@ def main(argv=None):
                     help="also time the matrix's data-movement floor (floor_kernel) --reps times")
     ap.add_argument("--no-spmv", action="store_true", help="skip the SpMV calls (with --floor)")
     ap.add_argument("--batched", default="",
-                    help="S,T,waves_per_block,grid,order: run the 256-matrix batch (fp64) with "
-                         "this schedule --reps times instead of the single-matrix workload")
+                    help="S,T,waves_per_block,grid,order[,x_stage]: run the 256-matrix batch "
+                         "(fp64) with this schedule --reps times instead of the single-matrix "
+                         "workload")
     a = ap.parse_args(argv)
     if a.batched:
         from . import batched

@@ -92,14 +92,20 @@ typedef struct {
 typedef struct bhs_batch_opaque *bhs_batch_t;
 
 /* Schedule of a batch: tile shape of the segmented sum, waves per block,
- * persistent grid size (0 = about 16 resident waves per CU) and item order
- * (0 = matrix order, 1 = most work first). */
+ * persistent grid size (0 = about 16 resident waves per CU), item order
+ * (0 = matrix order, 1 = most work first) and x staging: 0, or (fp64 with
+ * items_per_lane 4 only) 2 or 4 -- matrices with an even number of
+ * right-hand sides load each tile's x values into LDS, x_stage 16-byte
+ * chunks (2 right-hand sides each) per nonzero at a time, one cache request
+ * per nonzero and pass instead of one per pair; the arithmetic, and y, are
+ * the same.  Adds 1 KB + 4 KB * x_stage of LDS per wave. */
 typedef struct {
     int items_per_lane;
     int tiles_per_wave;
     int waves_per_block;
     int grid_blocks;
     int order;
+    int x_stage;
 } bhs_batch_schedule;
 
 /* ---- library-wide ------------------------------------------------------ */

@@ -46,8 +46,11 @@ def check_oracle(mats, ks, Xs, Ys, dtype, every=1):
             assert e <= TOL[dtype], (A.name, r, e)
 
 
-@pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("sched", [(8, 4, 4, 0, 1), (4, 1, 1, 0, 0), (16, 16, 2, 37, 1)])
+@pytest.mark.parametrize("dtype,sched", [
+    (dt, sc) for dt in (np.float64, np.float32)
+    for sc in ((8, 4, 4, 0, 1), (4, 1, 1, 0, 0), (16, 16, 2, 37, 1))] + [
+    (np.float64, (4, 4, 1, 0, 1, 2)), (np.float64, (4, 16, 1, 0, 1, 4)),
+    (np.float64, (4, 1, 2, 0, 0, 4))])
 def test_batched_equals_plain_and_oracle(dtype, sched):
     mats, ks, Xs = small_batch(7, dtype)
     say(f"small batch {np.dtype(dtype).name} schedule={sched}")
@@ -74,7 +77,11 @@ def test_odd_and_even_rhs_counts(dtype):
     mats, ks, Xs = small_batch(13, dtype, n=16, k_choices=(2, 3, 5, 6, 7, 16))
     say(f"odd/even k batch {np.dtype(dtype).name} ks={ks}")
     b = batched.SpmvBatch(mats, ks, dtype)
-    for sched in ((8, 4, 4, 0, 1), (4, 2, 2, 0, 0)):
+    scheds = [(8, 4, 4, 0, 1), (4, 2, 2, 0, 0)]
+    if dtype == np.float64:                 # x staged in LDS: one to four passes per tile
+        scheds += [(4, 4, 1, 0, 1, 4), (4, 2, 1, 0, 0, 4), (4, 4, 2, 0, 1, 2),
+                   (4, 2, 1, 0, 0, 2)]
+    for sched in scheds:
         b.set_schedule(*sched)
         b.set_x(Xs)
         b.run(1.0)
@@ -84,6 +91,26 @@ def test_odd_and_even_rhs_counts(dtype):
         for i, (yb, yp) in enumerate(zip(Yb, Yp)):
             assert np.array_equal(yb, yp), (sched, i, ks[i], mats[i].name)
         check_oracle(mats, ks, Xs, Yb, dtype)
+    b.close()
+
+
+def test_x_stage_schedules_are_checked():
+    """x staging runs on fp64 tiles of 4 items per lane only, in passes of 2 or
+    4 chunks, within the block's LDS; anything else is refused with an
+    error."""
+    from benchmark_spmv_using_csr_amd._lib import BhsError
+    mats, ks, Xs = small_batch(5, np.float64, n=4)
+    b = batched.SpmvBatch(mats, ks, np.float64)
+    for bad in ((8, 4, 1, 0, 1, 2), (4, 4, 1, 0, 1, 8), (4, 4, 8, 0, 1, 4)):
+        with pytest.raises(BhsError):
+            b.set_schedule(*bad)
+    b.set_schedule(4, 4, 1, 0, 1, 4)
+    assert b.schedule() == (4, 4, 1, 0, 1, 4)
+    b.close()
+    mats, ks, Xs = small_batch(5, np.float32, n=4)
+    b = batched.SpmvBatch(mats, ks, np.float32)
+    with pytest.raises(BhsError):
+        b.set_schedule(4, 4, 1, 0, 1, 4)
     b.close()
 
 
@@ -134,6 +161,14 @@ def test_full_256_matrix_batch():
     Yp = b.y(plain=True)
     assert all(np.array_equal(g, r) for g, r in zip(Yb, Yp))
     check_oracle(mats, ks, Xs, Yb, np.float64, every=16)
+    b.set_schedule(4, 16, 1, 0, 1, 0)
+    b.run_plain(1.0)
+    Yp4 = b.y(plain=True)
+    for xs in (4, 2):                       # x staged in LDS: the plain launches' bits
+        say(f"256-matrix batch x_stage={xs}")
+        b.set_schedule(4, 16, 1, 0, 1, xs)
+        b.run(1.0)
+        assert all(np.array_equal(g, r) for g, r in zip(b.y(), Yp4))
     b.close()
 
 
