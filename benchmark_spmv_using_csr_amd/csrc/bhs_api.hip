@@ -41,6 +41,16 @@ int set_error(int code, const std::string &msg)
     return code;
 }
 
+// Events used only for timing are created without the system-scope fence
+// that a default event performs when it is recorded (a cache writeback and
+// invalidation inside every timed interval: 4.3 us for an empty interval
+// between two default events, 3.2 us without it).  The results a timed call
+// leaves are read only after a stream synchronisation.
+inline hipError_t create_timing_event(hipEvent_t *e)
+{
+    return hipEventCreateWithFlags(e, hipEventDisableSystemFence);
+}
+
 #define BHS_HIP(call)                                                                    \
     do {                                                                                 \
         hipError_t e_ = (call);                                                          \
@@ -192,7 +202,7 @@ int init_platform(bhs_handle_t h, int device)
     im->max_lds = std::min<size_t>(prop.sharedMemPerBlock, 65536);
     if (!im->platform) {
         BHS_HIP(hipStreamCreateWithFlags(&im->stream, hipStreamNonBlocking));
-        for (auto &e : im->ev) BHS_HIP(hipEventCreate(&e));
+        for (auto &e : im->ev) BHS_HIP(create_timing_event(&e));
         BHS_HIP(hipMalloc(&im->d_counters, 8 * sizeof(unsigned long long)));
     }
     im->device = device;
@@ -1522,7 +1532,8 @@ int bhs_stream_copy(int device, long long bytes, int reps, double *ms)
     if ((e = hipMalloc(&a, (size_t)n16 * 16)) != hipSuccess) fail(e, "hipMalloc");
     else if ((e = hipMalloc(&b, (size_t)n16 * 16)) != hipSuccess) fail(e, "hipMalloc");
     else if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) fail(e, "stream");
-    else if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess)
+    else if ((e = create_timing_event(&e0)) != hipSuccess ||
+             (e = create_timing_event(&e1)) != hipSuccess)
         fail(e, "event");
     else if ((e = hipMemsetAsync(a, 0, (size_t)n16 * 16, st)) != hipSuccess) fail(e, "memset");
     else {

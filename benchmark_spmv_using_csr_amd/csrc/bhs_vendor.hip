@@ -71,8 +71,10 @@ int csrmv_bench(int device, int m, int n, int nnz, const int *row_ptr, const int
     Buffers b;
     VH(hipSetDevice(device));
     VH(hipStreamCreate(&b.stream));
-    VH(hipEventCreate(&b.e0));
-    VH(hipEventCreate(&b.e1));
+    // timed the same way as the library's own calls: events without the
+    // system-scope fence of a default event
+    VH(hipEventCreateWithFlags(&b.e0, hipEventDisableSystemFence));
+    VH(hipEventCreateWithFlags(&b.e1, hipEventDisableSystemFence));
     VH(hipMalloc((void **)&b.rp, sizeof(int) * (size_t)(m + 1)));
     VH(hipMalloc((void **)&b.col, sizeof(int) * (size_t)(nnz > 0 ? nnz : 1)));
     VH(hipMalloc(&b.val, sizeof(T) * (size_t)(nnz > 0 ? nnz : 1)));
