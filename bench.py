@@ -198,19 +198,6 @@ OTHER_WORKLOADS = (("power-law", "f32"), ("laplace", "f64"), ("block3", "f64"),
                    ("uniform", "f64"))
 
 
-def workload(name):
-    """The synthetic matrices and their seeded x vectors."""
-    if name == "power-law":
-        return matgen.power_law_1m(), matgen.vector(102, 1 << 20)
-    if name == "uniform":
-        return matgen.uniform_65536(), matgen.vector(101, 65536)
-    if name == "laplace":
-        A = matgen.laplacian_160()
-        return A, matgen.vector(103, A.n)
-    A = matgen.block3_2m()
-    return A, matgen.vector(104, A.n)
-
-
 def run_others(flush_bytes, vendor_cols, reps=30):
     """The other benchmark matrices (and fp32 on the power-law one), each with
     the learned tuner's shortlisted tile, timed cold like the headline, beside
@@ -221,7 +208,7 @@ def run_others(flush_bytes, vendor_cols, reps=30):
     rows = []
     for wl, dt in OTHER_WORKLOADS:
         dtype = np.float64 if dt == "f64" else np.float32
-        A, x = workload(wl)
+        A, x = matgen.workload(wl)
         op = bhsparse_spmv(dtype)
         _lib.check(op.init_platform(0), "init_platform")
         y = np.zeros(A.m, dtype)
@@ -333,8 +320,7 @@ def main():
     ap.add_argument("--no-vendor", action="store_true")
     ap.add_argument("--no-others", action="store_true",
                     help="skip the other matrices (fp32 power-law, Laplacian, block3, uniform)")
-    ap.add_argument("--workload", choices=["power-law", "uniform", "laplace", "block3"],
-                    default="power-law")
+    ap.add_argument("--workload", choices=list(matgen.WORKLOADS), default="power-law")
     ap.add_argument("--mtx", default="",
                     help="Matrix Market file to use instead of a synthetic workload "
                          "(sorted, duplicates merged; x uniform in [-1, 1])")
@@ -357,7 +343,7 @@ def main():
         A.name = f"Matrix Market file {os.path.basename(a.mtx)} m={A.m} n={A.n}"
         x = matgen.vector(105, A.n)
     else:
-        A, x = workload(a.workload)
+        A, x = matgen.workload(a.workload)
 
     managed = a.memory != "device"
     r = run_gpu(A, x, dtype, cfg, a.warmup, a.steps, a.flush_mb << 20,

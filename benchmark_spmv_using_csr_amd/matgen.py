@@ -145,6 +145,24 @@ def block3_2m(seed: int = 3) -> CSR:
     return A
 
 
+WORKLOADS = ("power-law", "uniform", "laplace", "block3")
+
+
+def workload(name: str):
+    """One of the benchmark matrices by name, with its seeded x vector."""
+    if name == "power-law":
+        return power_law_1m(), vector(102, 1 << 20)
+    if name == "uniform":
+        return uniform_65536(), vector(101, 65536)
+    if name == "laplace":
+        A = laplacian_160()
+        return A, vector(103, A.n)
+    if name == "block3":
+        A = block3_2m()
+        return A, vector(104, A.n)
+    raise ValueError(f"unknown workload {name!r}; one of {WORKLOADS}")
+
+
 # ---- tuner and batch families ---------------------------------------------
 
 def _rng(seed):

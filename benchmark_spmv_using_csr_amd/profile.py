@@ -19,22 +19,9 @@ from . import _lib, matgen
 from .spmv import bhsparse_spmv, stream_copy
 
 
-def workload(name):
-    if name == "power-law":
-        return matgen.power_law_1m(), matgen.vector(102, 1 << 20)
-    if name == "uniform":
-        return matgen.uniform_65536(), matgen.vector(101, 65536)
-    if name == "laplace":
-        A = matgen.laplacian_160()
-        return A, matgen.vector(103, A.n)
-    A = matgen.block3_2m()
-    return A, matgen.vector(104, A.n)
-
-
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="power-law",
-                    choices=["power-law", "uniform", "laplace", "block3"])
+    ap.add_argument("--workload", default="power-law", choices=list(matgen.WORKLOADS))
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
     ap.add_argument("--config", default="", help="S,waves_per_block,tiles_per_wave")
     ap.add_argument("--variant", type=int, default=0,
@@ -67,7 +54,7 @@ def main(argv=None):
             print(f"copy {a.copy_mb} MiB: median {np.median(cm) * 1e3:.2f} us", flush=True)
         return 0
     dtype = np.float64 if a.dtype == "f64" else np.float32
-    A, x = workload(a.workload)
+    A, x = matgen.workload(a.workload)
     op = bhsparse_spmv(dtype)
     _lib.check(op.init_platform(0), "init_platform")
     if a.config:

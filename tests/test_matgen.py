@@ -66,3 +66,16 @@ def test_families(fam):
     A = matgen.family_matrix(100 + fam, 1e3, 2e4, fam)
     _check_csr(A)
     assert A.m >= 1000
+
+
+def test_named_workloads_are_the_documented_matrices():
+    """The benchmark matrices by name (bench.py --workload, the profile driver):
+    sizes as the workload descriptions state, and x of length n."""
+    import pytest
+    from benchmark_spmv_using_csr_amd import matgen
+    A, x = matgen.workload("uniform")
+    assert (A.m, A.n) == (65536, 65536) and len(x) == A.n
+    A, x = matgen.workload("laplace")
+    assert A.m == 160 ** 3 and A.nnz == 7 * 160 ** 3 - 6 * 160 ** 2 and len(x) == A.n
+    with pytest.raises(ValueError):
+        matgen.workload("nope")
