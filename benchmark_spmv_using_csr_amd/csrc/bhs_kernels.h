@@ -481,7 +481,7 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
     unsigned long long n_fast = 0, n_dirty = 0;
 
     // Row chunk of the next tile loaded one tile ahead; the next tile's stream
-    // is in flight while this one is reduced (the last tile reloads itself).
+    // is in flight while this one is reduced.
     Tile<T, S> cur;
     if (BOUNDED && E0 + P > mv.nnz) tile_load_checked<T, S>(mv.col, mv.val, E0, lane, mv.nnz, cur);
     else tile_load<T, S>(mv.col, mv.val, E0, lane, cur);
@@ -513,9 +513,11 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
         rc = load_chunk<CP>(row_ptr, r1, lane, mv.m);
         __builtin_amdgcn_sched_barrier(0);
         Tile<T, S> nxt;
-        const int Bn = (p + 1 < p_end) ? B + P : B;
-        if (BOUNDED && Bn + P > mv.nnz) tile_load_checked<T, S>(mv.col, mv.val, Bn, lane, mv.nnz, nxt);
-        else tile_load<T, S>(mv.col, mv.val, Bn, lane, nxt);
+        if (p + 1 < p_end) {                      // wave-uniform; the last tile loads nothing more
+            const int Bn = B + P;
+            if (BOUNDED && Bn + P > mv.nnz) tile_load_checked<T, S>(mv.col, mv.val, Bn, lane, mv.nnz, nxt);
+            else tile_load<T, S>(mv.col, mv.val, Bn, lane, nxt);
+        }
 
         const bool cont = read_lane(cc.a[0], 0) < B;   // item 0 continues row r0 (lane 0 holds row r0)
         if (p == p_begin) syncing = cont;
