@@ -510,7 +510,7 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
             xr.first(x, cur, ks);
         }
         const RowChunk<CP> cc = rc;
-        rc = load_chunk<CP>(row_ptr, r1, lane, mv.m);
+        if (p + 1 < p_end) rc = load_chunk<CP>(row_ptr, r1, lane, mv.m);
         __builtin_amdgcn_sched_barrier(0);
         Tile<T, S> nxt;
         if (p + 1 < p_end) {                      // wave-uniform; the last tile loads nothing more
