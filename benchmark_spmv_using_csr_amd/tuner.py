@@ -38,7 +38,7 @@ DEFAULT_NET = os.path.join(DATA_DIR, "tuner_net.json")
 # column-panel copy of A
 CLASSES = [(0, S, wpb, T, 0) for S in (4, 8, 16) for wpb in (1, 4) for T in (1, 4, 8, 16)] + \
           [(1, 8, 4, 1, tpv) for tpv in (2, 4, 8, 16, 32, 64)] + \
-          [(2, S, 1, T, npan) for npan in (2, 4) for S in (4, 8) for T in (4, 8, 16)]
+          [(2, S, 1, T, npan) for npan in (2, 4) for S in (4, 8) for T in (2, 4, 8, 16)]
 K = len(CLASSES)
 # The method without the tuner: one fixed tile for every matrix.
 DEFAULT_CLASS = CLASSES.index((0, 8, 4, 4, 0))
