@@ -214,10 +214,14 @@ def run_others(flush_bytes, vendor_cols, reps=30):
         y = np.zeros(A.m, dtype)
         _lib.check(op.prepare_mem(A.m, A.n, A.nnz, A.row_ptr, A.col, A.val.astype(dtype),
                                   x.astype(dtype), y), "prepare_mem")
-        cfg = tuner.CLASSES[op.apply_tuner(net, tuner.CLASSES, topk=TUNER_TOPK, reps=7,
-                                           flush_bytes=flush_bytes)]
+        top1 = op.apply_tuner(net, tuner.CLASSES)          # the network's own pick
+        chosen = op.apply_tuner(net, tuner.CLASSES, topk=TUNER_TOPK, reps=7,
+                                flush_bytes=flush_bytes)
+        cfg = tuner.CLASSES[chosen]
         tot, main_ms = op.bench(1.0, warmup=5, reps=reps, flush_bytes=flush_bytes)
         _lib.check(op.get_y(), "get_y")
+        # the exhaustive autotuner as the yardstick of both picks
+        us, best = op.autotune(tuner.CLASSES, reps=7, flush_bytes=flush_bytes)
         op.close()
         b = eff_bytes(A.m, A.n, A.nnz, np.dtype(dtype).itemsize)
         med, km = median(tot), float(np.mean(main_ms))
@@ -226,7 +230,13 @@ def run_others(flush_bytes, vendor_cols, reps=30):
                "gbps": round(b / (med * 1e-3) / 1e9, 1),
                "kernel_mean_us": round(km * 1e3, 3),
                "roofline_frac": round(b / (km * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
-               "whole_call_frac": round(b / (med * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)}
+               "whole_call_frac": round(b / (med * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+               "tuner": {"network_top1": tuner.class_name(tuner.CLASSES[top1]),
+                         "network_top1_regret_vs_exhaustive":
+                             round(float(us[top1]) / float(us[best]) - 1, 4),
+                         "exhaustive_best": tuner.class_name(tuner.CLASSES[best]),
+                         "regret_vs_exhaustive":
+                             round(float(us[chosen]) / float(us[best]) - 1, 4)}}
         if vendor_cols:
             v = run_vendor(A, x, dtype, flush_bytes, y, reps=10)
             row["vendor_best"] = {"alg": v["best_alg"], "cold_us": v["best_cold_us"],
