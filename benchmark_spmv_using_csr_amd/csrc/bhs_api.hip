@@ -689,14 +689,18 @@ void launch_segsum_s(Impl<T> *im, T alpha, unsigned long long *stats, hipEvent_t
     q.alpha = alpha;
     const int blocks = (im->NW + q.wpb - 1) / q.wpb;
     const size_t lds = (size_t)q.wpb * q.wave_lds_bytes;
+    // the stacked matrix of the column-panel layout has no empty rows
+    auto kern = im->cfg.variant == BHS_VARIANT_SEGSUM_PANELS
+                    ? bhs::segsum_kernel<T, S, STATS, CP, false, true>
+                    : bhs::segsum_kernel<T, S, STATS, CP, false, false>;
     if (e0 && e1)
-        hipExtLaunchKernelGGL((bhs::segsum_kernel<T, S, STATS, CP>), dim3(blocks), dim3(64 * q.wpb),
-                              (std::uint32_t)lds, im->stream, e0, e1, 0u, L.row_ptr, L.col, L.val,
-                              (const T *)im->d_x, L.y, (const int *)im->d_tile_row,
-                              im->d_sync_row, im->d_sync_val, stats, q);
+        hipExtLaunchKernelGGL(kern, dim3(blocks), dim3(64 * q.wpb), (std::uint32_t)lds, im->stream,
+                              e0, e1, 0u, L.row_ptr, L.col, L.val, (const T *)im->d_x, L.y,
+                              (const int *)im->d_tile_row, im->d_sync_row, im->d_sync_val, stats,
+                              q);
     else
-        hipLaunchKernelGGL((bhs::segsum_kernel<T, S, STATS, CP>), dim3(blocks), dim3(64 * q.wpb), lds,
-                           im->stream, L.row_ptr, L.col, L.val, im->d_x, L.y, im->d_tile_row,
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * q.wpb), lds, im->stream, L.row_ptr, L.col,
+                           L.val, (const T *)im->d_x, L.y, (const int *)im->d_tile_row,
                            im->d_sync_row, im->d_sync_val, stats, q);
 }
 

@@ -408,8 +408,10 @@ struct RhsTag {
 // XCH > 0 (fp64, S = 4, KMAX > 1): right-hand-side pairs take their x values
 // from the tile staged in LDS, XCH chunks per item at a time (XStage), at
 // s_stage: 1 KB of column indices, then the staging buffer.
+// NOEMPTY: the matrix has no empty rows (the stacked matrix of the column-panel
+// layout), so no tile is dirty and the empty-row paths are left out.
 template <typename T, int S, int KMAX, bool STRIDED, bool STATS, int CP, bool BOUNDED = false,
-          int XCH = 0>
+          int XCH = 0, bool NOEMPTY = false>
 __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, const int lane,
                                             unsigned *s_head, T *s_buf, T *s_carry, const T alpha,
                                             unsigned long long *stats,
@@ -433,7 +435,7 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
     T *__restrict__ y = mv.y;
 
     // Rows before the first nonzero belong to no tile: spread their zeroing.
-    if (mv.lead_empty > 0) {
+    if (!NOEMPTY && mv.lead_empty > 0) {
         const long long lo = (long long)mv.lead_empty * w / mv.NW;
         const long long hi = (long long)mv.lead_empty * (w + 1) / mv.NW;
         for (long long r = lo + lane; r < hi; r += kWave)
@@ -594,9 +596,9 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                     if (q > 0 && r0 + q * kWave > stop) return;   // wave-uniform: no rows here
                     // the shift reads the next lane: it runs on the whole
                     // wave, before any lane-dependent condition masks lanes off
-                    const int next = chunk_next<q>(cc, lane);
                     const bool have = r0 + q * kWave + lane <= stop;
-                    const bool ne = have && next > cc.a[q];
+                    bool ne = have;
+                    if constexpr (!NOEMPTY) ne = have && chunk_next<q>(cc, lane) > cc.a[q];
                     lane_dirty = lane_dirty || (have && !ne);
                     mne[q] = ballot(ne);
                     if (ne) {
@@ -607,7 +609,16 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                 if constexpr (BIG)
                 for (int rb = r0 + CR; rb <= stop;) {   // >= CR rows
                     const int rq = min(rb + lane, stop);
-                    const int rp = row_ptr[rq], rpn = row_ptr[rq + 1];
+                    const int rp = row_ptr[rq];
+                    if constexpr (NOEMPTY) {
+                        if (rb + lane <= stop) {
+                            const int o = rp - B;
+                            if (o >= 0 && o < P) atomicOr(&s_head[o / S], 1u << (o % S));
+                        }
+                        rb += kWave;
+                        continue;
+                    }
+                    const int rpn = row_ptr[rq + 1];
                     const int vb = row_ptr[min(rb + kWave, stop + 1)];
                     if (rb + kWave <= stop && read_lane(rp, 0) == vb) {
                         // 64 empty rows with more rows after them: skip the whole run
@@ -629,7 +640,7 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
                     }
                     rb += kWave;
                 }
-                const bool dirty = ballot(lane_dirty) != 0ull;
+                const bool dirty = !NOEMPTY && ballot(lane_dirty) != 0ull;
                 wave_sync();
                 unsigned hb = s_head[lane];
                 if (lane == 0) hb |= 1u;
@@ -825,7 +836,7 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
 
 // q.lead_empty < 0: read the number of leading empty rows from tile_row[0]
 // (the first tile's row), for launches that have not copied it to the host.
-template <typename T, int S, bool STATS, int CP, bool BOUNDED = false>
+template <typename T, int S, bool STATS, int CP, bool BOUNDED = false, bool NOEMPTY = false>
 __global__ __launch_bounds__(512) void segsum_kernel(
     const int *__restrict__ row_ptr, const int *__restrict__ col, const T *__restrict__ val,
     const T *__restrict__ x, T *__restrict__ y, const int *__restrict__ tile_row,
@@ -842,7 +853,7 @@ __global__ __launch_bounds__(512) void segsum_kernel(
     const int lead_empty = q.lead_empty >= 0 ? q.lead_empty : tile_row[0];
     MatView<T> mv{row_ptr, tile_row, col, val, x, y, sync_row, sync_val,
                   q.m, q.nnz, q.NP, q.NW, lead_empty, 1, 1, q.TPW, 0};
-    segsum_wave<T, S, 1, false, STATS, CP, BOUNDED>(mv, w, lane, reinterpret_cast<unsigned *>(wave_lds),
+    segsum_wave<T, S, 1, false, STATS, CP, BOUNDED, 0, NOEMPTY>(mv, w, lane, reinterpret_cast<unsigned *>(wave_lds),
                                        reinterpret_cast<T *>(wave_lds + 256), nullptr, q.alpha,
                                        stats);
 }
