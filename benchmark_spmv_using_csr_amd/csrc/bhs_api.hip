@@ -457,19 +457,24 @@ int build_chains(Impl<T> *im, long long PW, long long NW)
     const std::vector<int> &rp = im->h_prow_ptr;
     const long long groups = 64LL * bhs::kCombineGroups;
     const long long n_cw = ((long long)im->m + groups - 1) / groups;
-    std::vector<bhs::ChainRec> rec((size_t)std::max<long long>(n_cw, 1), bhs::ChainRec{0, 0, 0, -1});
+    constexpr int K = bhs::kChainSlots;
+    // K slots per combine wave, then the records linked from its last slot
+    std::vector<bhs::ChainRec> rec((size_t)std::max<long long>(n_cw, 1) * K,
+                                   bhs::ChainRec{0, 0, 0, -1});
+    std::vector<unsigned char> used((size_t)std::max<long long>(n_cw, 1), 0);
     auto add = [&](int R, int first, int count) {
         const int i = im->h_prow_of[(size_t)R];
         const int pan = (int)(std::upper_bound(im->h_pan_start.begin(), im->h_pan_start.end(), R) -
                               im->h_pan_start.begin()) - 1;
         const long long cw = i / groups;
         bhs::ChainRec c{first, count, (int)(i % groups) << 4 | pan, -1};
-        if (rec[(size_t)cw].count == 0) {
-            rec[(size_t)cw] = c;
-        } else {                              // second in the combine wave's list
-            c.next = rec[(size_t)cw].next;
+        const size_t last = (size_t)(cw * K + K - 1);
+        if (used[(size_t)cw] < K) {
+            rec[(size_t)(cw * K + used[(size_t)cw]++)] = c;
+        } else {                              // beyond the slots: first in the linked list
+            c.next = rec[last].next;
             rec.push_back(c);
-            rec[(size_t)cw].next = (int)rec.size() - 1;
+            rec[last].next = (int)rec.size() - 1;
         }
     };
     int R_open = -1, first = 0, count = 0;

@@ -1055,11 +1055,16 @@ __global__ __launch_bounds__(256) void calibrate_batched_kernel(const int *__res
 // launch on this layout).  A stacked row whose nonzeros cross wave boundaries
 // has a chain of consecutive waves that each left a partial in sync_val; the
 // chains depend only on the stacked matrix and the tile shape, so they are
-// listed once per analysis as ChainRec records: chains[cw] is the first chain
-// landing in combine wave cw (count 0: none), `next` links the others.  The
-// chain's total is summed 64 entries at a time (wave_sum of each chunk, the
-// chunk sums in order) and added to that partial before the panel sum:
-// y_stack[R] + total, the value calibrate_kernel would have stored.
+// listed once per analysis as ChainRec records: combine wave cw has
+// kChainSlots slots chains[cw * kChainSlots + j] (count 0: empty), filled in
+// order, and the last slot's `next` links any further records.  The slots are
+// loaded with the masks, and the first 64 partials of every slot's chain with
+// the panel partials, so that a combine wave's chains add no round trip to
+// memory unless it has more than kChainSlots of them or one of more than 64
+// waves.  A chain's total is summed 64 entries at a time (wave_sum of each
+// chunk, the chunk sums in order) and added to that partial before the panel
+// sum: y_stack[R] + total, the value calibrate_kernel would have stored.  A
+// stacked row has at most one chain, so chains never meet in one partial.
 struct ChainRec {
     int first;   // first wave of the chain (its sync_val index)
     int count;   // waves in the chain (0: no record)
@@ -1067,6 +1072,7 @@ struct ChainRec {
     int next;    // index of the next record of the same combine wave, -1: none
 };
 constexpr int kCombineGroups = 2;   // 64-row groups per combine wave
+constexpr int kChainSlots = 4;      // chain records per combine wave loaded up front
 
 template <typename T, int NPAN, int GROUPS>
 __global__ __launch_bounds__(256) void combine_panels_kernel(const T *__restrict__ ystack,
@@ -1081,9 +1087,10 @@ __global__ __launch_bounds__(256) void combine_panels_kernel(const T *__restrict
         (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
     const long long g0 = (long long)cw * GROUPS;
     if (g0 * kWave >= m) return;                      // whole wave past the last row
-    ChainRec c = chains[cw];                          // wave-uniform
-    // masks and bases of every group first, so that no wait for a mask also
-    // waits for partials already in flight
+    // chain slots, masks and bases first, so that no wait for them also waits
+    // for partials already in flight
+    ChainRec slot{0, 0, 0, -1};
+    if (lane < kChainSlots) slot = chains[(long long)cw * kChainSlots + lane];
     unsigned mk[GROUPS];
     int gb[GROUPS][NPAN];
 #pragma unroll
@@ -1106,21 +1113,35 @@ __global__ __launch_bounds__(256) void combine_panels_kernel(const T *__restrict
             v[h][p] = has ? ystack[pos] : T(0);
         }
     }
-    while (c.count > 0) {                             // wave-uniform loop
-        T tot = T(0);
-        for (int j = 0; j < c.count; j += kWave) {
-            const T s = wave_sum(j + lane < c.count ? sync_val[(long long)c.first + j + lane]
-                                                    : T(0));
-            tot = j == 0 ? s : tot + s;
-        }
-        const int slot = c.where >> 4, pan = c.where & 15;
+    // the first chunk of every slot's chain, in flight with the partials
+    int c_first[kChainSlots], c_count[kChainSlots], c_where[kChainSlots];
+    T c0[kChainSlots];
+#pragma unroll
+    for (int j = 0; j < kChainSlots; ++j) {
+        c_first[j] = read_lane(slot.first, j);
+        c_count[j] = read_lane(slot.count, j);
+        c_where[j] = read_lane(slot.where, j);
+        c0[j] = lane < c_count[j] ? sync_val[(long long)c_first[j] + lane] : T(0);
+    }
+    auto add_chain = [&](int first, int count, int where, T s) {
+        T tot = wave_sum(s);
+        for (int j = kWave; j < count; j += kWave)
+            tot = tot + wave_sum(j + lane < count ? sync_val[(long long)first + j + lane] : T(0));
+        const int row = where >> 4, pan = where & 15;
 #pragma unroll
         for (int h = 0; h < GROUPS; ++h)
 #pragma unroll
             for (int p = 0; p < NPAN; ++p)
-                if (pan == p && slot == h * kWave + lane) v[h][p] = v[h][p] + tot;
-        if (c.next < 0) break;
-        c = chains[c.next];
+                if (pan == p && row == h * kWave + lane) v[h][p] = v[h][p] + tot;
+    };
+#pragma unroll
+    for (int j = 0; j < kChainSlots; ++j)
+        if (c_count[j] > 0) add_chain(c_first[j], c_count[j], c_where[j], c0[j]);
+    for (int nx = read_lane(slot.next, kChainSlots - 1); nx >= 0;) {   // wave-uniform
+        const ChainRec c = chains[nx];
+        add_chain(c.first, c.count, c.where,
+                  lane < c.count ? sync_val[(long long)c.first + lane] : T(0));
+        nx = c.next;
     }
 #pragma unroll
     for (int h = 0; h < GROUPS; ++h) {

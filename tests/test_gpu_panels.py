@@ -105,6 +105,34 @@ def test_wave_chains_longer_than_a_chunk(cfg):
     assert np.array_equal(y, yr)
 
 
+def many_chains_case(seed, wave_nnz, npan):
+    """Ten neighbouring rows, each about 2.5 waves long in every panel, among
+    short random rows: one combine wave receives 10 x npan chains, more than
+    the slots it loads up front, so most of them come from the linked list."""
+    rng = np.random.default_rng(seed)
+    n = 40000
+    lens = list(rng.integers(0, 12, 600))
+    for i in range(10, 20):
+        lens[i] = int(2.5 * wave_nnz * npan) + i
+    rp = np.zeros(len(lens) + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.choice(n, L, replace=False)) for L in lens]).astype(np.int32)
+    val = rng.integers(-8, 9, int(rp[-1])).astype(np.float64)
+    x = rng.integers(-8, 9, n).astype(np.float64)
+    return cases.Case("many chains", len(lens), n, rp.astype(np.int32), col, val, x)
+
+
+@pytest.mark.parametrize("cfg", [(4, 1, 1, 8), (8, 2, 1, 4), (4, 1, 2, 2)])
+def test_more_chains_than_slots_in_one_combine_wave(cfg):
+    S, _, tpw, npan = cfg
+    c = many_chains_case(S + npan, 64 * S * tpw, npan)
+    say(f"panels many chains cfg={cfg} nnz={c.nnz}")
+    y, op = run(c, np.float64, cfg)
+    op.close()
+    assert np.array_equal(y, model(c, np.float64, cfg))
+    assert np.array_equal(y, oracle.spmv_serial(c.row_ptr, c.col, c.val, c.x))
+
+
 @pytest.mark.parametrize("npan", [2, 4, 8])
 def test_partition_of_the_stacked_matrix(npan):
     A = matgen.power_law(11, 20000, 20000, 1.8, 5000, 0.1, 0)
