@@ -32,7 +32,7 @@ from benchmark_spmv_using_csr_amd.spmv import (bhsparse_spmv, device_synchronize
 HBM_PEAK_GBPS = 8000.0
 # the tile comes from the learned tuner's shortlist: its TUNER_TOPK best-predicted
 # classes are timed and the fastest is kept
-TUNER_TOPK = 3
+TUNER_TOPK = 5
 METRIC = "effective GB/s (% HBM peak) CSR SpMV; tuned+batched vs plain speedup at equal err"
 
 
