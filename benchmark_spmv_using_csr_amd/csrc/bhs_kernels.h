@@ -482,8 +482,13 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
     };
     unsigned long long n_fast = 0, n_dirty = 0;
 
-    // Row chunk of the next tile loaded one tile ahead; the next tile's stream
-    // is in flight while this one is reduced.
+    // Row chunk of the next tile loaded one tile ahead.  The next tile's
+    // stream is in flight while this one is reduced (kPrefetch) in fp32 and with
+    // several right-hand sides.  In fp64 with one right-hand side a tile is
+    // loaded at the top of its own iteration: there the prefetched tile holds
+    // 24 more VGPRs, and the four-panel power-law tiles ran 0.4-1.9 us slower
+    // with it (fp32 0.6 us faster, the plain and banded tiles alike).
+    constexpr bool kPrefetch = KMAX > 1 || sizeof(T) < 8;
     Tile<T, S> cur;
     if (BOUNDED && E0 + P > mv.nnz) tile_load_checked<T, S>(mv.col, mv.val, E0, lane, mv.nnz, cur);
     else tile_load<T, S>(mv.col, mv.val, E0, lane, cur);
@@ -496,6 +501,12 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
         const int r0 = r1;
         r1 = tile_row[p + 1];
         const bool general = r0 != r1;
+        if constexpr (!kPrefetch) {
+            if (p != p_begin) {
+                if (BOUNDED && B + P > mv.nnz) tile_load_checked<T, S>(mv.col, mv.val, B, lane, mv.nnz, cur);
+                else tile_load<T, S>(mv.col, mv.val, B, lane, cur);
+            }
+        }
         RhsX<T, S, KMAX> xr;
         [[maybe_unused]] RhsPair<T, S> xp;
         if constexpr (kStage) {
@@ -514,8 +525,8 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
         const RowChunk<CP> cc = rc;
         if (p + 1 < p_end) rc = load_chunk<CP>(row_ptr, r1, lane, mv.m);
         __builtin_amdgcn_sched_barrier(0);
-        Tile<T, S> nxt;
-        if (p + 1 < p_end) {                      // wave-uniform; the last tile loads nothing more
+        [[maybe_unused]] Tile<T, S> nxt;
+        if (kPrefetch && p + 1 < p_end) {         // wave-uniform; the last tile loads nothing more
             const int Bn = B + P;
             if (BOUNDED && Bn + P > mv.nnz) tile_load_checked<T, S>(mv.col, mv.val, Bn, lane, mv.nnz, nxt);
             else tile_load<T, S>(mv.col, mv.val, Bn, lane, nxt);
@@ -807,7 +818,7 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
             if (r1 - r0 < CR) general_tile(BigTag<false>{});
             else general_tile(BigTag<true>{});
         }
-        cur = nxt;
+        if constexpr (kPrefetch) cur = nxt;
     }
 
     if (open) {
