@@ -783,15 +783,18 @@ void launch_csrvec_t(Impl<T> *im, T alpha, hipEvent_t e0, hipEvent_t e1)
     const long long vpb = threads / TPV;
     long long blocks = ((long long)im->m + vpb - 1) / vpb;
     blocks = std::max<long long>(1, std::min<long long>(blocks, 256LL * 32));
+    // x is read ahead into each XCD's L2 when it takes at most 1 MB there
+    const long long x_bytes = (long long)im->n * (long long)sizeof(T);
+    const long long x_lines = x_bytes <= (1LL << 20) ? (x_bytes + 127) / 128 : 0;
     if (e0 && e1)
         hipExtLaunchKernelGGL((bhs::csrvec_kernel<T, TPV>), dim3((unsigned)blocks), dim3(threads),
                               0u, im->stream, e0, e1, 0u, im->m, (const int *)im->d_row_ptr,
                               (const int *)im->d_col, (const T *)im->d_val, (const T *)im->d_x,
-                              im->d_y, alpha);
+                              im->d_y, alpha, x_lines);
     else
         hipLaunchKernelGGL((bhs::csrvec_kernel<T, TPV>), dim3((unsigned)blocks), dim3(threads), 0,
                            im->stream, im->m, im->d_row_ptr, im->d_col, im->d_val, im->d_x,
-                           im->d_y, alpha);
+                           im->d_y, alpha, x_lines);
 }
 
 template <typename T>
