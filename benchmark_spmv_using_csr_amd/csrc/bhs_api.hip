@@ -692,6 +692,9 @@ void launch_segsum_s(Impl<T> *im, T alpha, unsigned long long *stats, hipEvent_t
     q.wpb = im->cfg.waves_per_block;
     q.wave_lds_bytes = (256 + 64 * S * (int)sizeof(T) + 15) / 16 * 16;
     q.alpha = alpha;
+    // x is read ahead into each XCD's L2 when it takes at most 1 MB there
+    const long long x_bytes = (long long)im->n * (long long)sizeof(T);
+    q.x_lines = x_bytes <= (1LL << 20) ? (x_bytes + 127) / 128 : 0;
     const int blocks = (im->NW + q.wpb - 1) / q.wpb;
     const size_t lds = (size_t)q.wpb * q.wave_lds_bytes;
     // the stacked matrix of the column-panel layout has no empty rows
@@ -1457,6 +1460,7 @@ int spmv_noalloc(int m, int n, int nnz, const int *row_ptr, const int *col, cons
     q.wpb = WPB;
     q.wave_lds_bytes = (256 + 64 * S * (int)sizeof(T) + 15) / 16 * 16;
     q.alpha = alpha;
+    q.x_lines = 0;
     hipLaunchKernelGGL((bhs::segsum_kernel<T, S, false, 2, true>),
                        dim3((unsigned)((NW + WPB - 1) / WPB)), dim3(64 * WPB),
                        (size_t)WPB * q.wave_lds_bytes, st, row_ptr, col, val, x, y,

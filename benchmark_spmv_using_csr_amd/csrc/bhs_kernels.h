@@ -46,6 +46,7 @@ template <typename T>
 struct SegsumParams {
     int m, nnz, NP, TPW, NW, lead_empty, wpb, wave_lds_bytes;
     T alpha;
+    long long x_lines;   // > 0: x is read ahead into each XCD's L2 (x_lines 128-byte lines)
 };
 
 // One tile's column indices and values in registers.  Loads are coalesced:
@@ -415,7 +416,7 @@ template <typename T, int S, int KMAX, bool STRIDED, bool STATS, int CP, bool BO
 __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, const int lane,
                                             unsigned *s_head, T *s_buf, T *s_carry, const T alpha,
                                             unsigned long long *stats,
-                                            unsigned char *s_stage = nullptr)
+                                            unsigned char *s_stage = nullptr, int touch = 0)
 {
     constexpr int P = kWave * S;
     constexpr int G = Tile<T, S>::G;
@@ -495,6 +496,9 @@ __device__ __forceinline__ void segsum_wave(const MatView<T> &mv, const int w, c
     constexpr int CR = CP * kWave;              // rows a chunk covers
     RowChunk<CP> rc = load_chunk<CP>(row_ptr, r_first, lane, mv.m);
     int r1 = r_first;
+    // the caller's read-ahead of x (issued before the loads above) is
+    // consumed here, so its register is free during the tile loop
+    asm volatile("" ::"v"(touch));   // emits no instruction
 
     for (int p = p_begin; p < p_end; ++p) {
         const int B = p * P;
@@ -856,6 +860,14 @@ __global__ __launch_bounds__(512) void segsum_kernel(
 {
     extern __shared__ double smem_raw[];
     const int lane = lane_id();
+    // q.x_lines > 0 (x small enough for every L2): the blocks of each XCD
+    // (dealt round-robin, block b on XCD b % 8) read one word of every
+    // 128-byte line of x between them, so the first tile's gathers hit L2.
+    int touch = 0;
+    {
+        const long long t = (long long)(blockIdx.x / 8) * blockDim.x + threadIdx.x;
+        if (t < q.x_lines) touch = reinterpret_cast<const int *>(x)[t * (128 / sizeof(int))];
+    }
     const int wib = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int blk = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int w = blk * q.wpb + wib;
@@ -866,7 +878,7 @@ __global__ __launch_bounds__(512) void segsum_kernel(
                   q.m, q.nnz, q.NP, q.NW, lead_empty, 1, 1, q.TPW, 0};
     segsum_wave<T, S, 1, false, STATS, CP, BOUNDED, 0, NOEMPTY>(mv, w, lane, reinterpret_cast<unsigned *>(wave_lds),
                                        reinterpret_cast<T *>(wave_lds + 256), nullptr, q.alpha,
-                                       stats);
+                                       stats, nullptr, touch);
 }
 
 // ---------------------------------------------------------------------------
