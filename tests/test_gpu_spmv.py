@@ -221,6 +221,32 @@ def _full(name):
     return A, x
 
 
+@pytest.mark.parametrize("n", [33, 131072, 131073, 262144])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_x_read_ahead_sizes(n, dtype):
+    """x at and around the largest size the kernels read ahead into L2 (1 MB:
+    131,072 doubles or 262,144 floats), and a tiny one; the first and the last
+    column are in use.  y within the oracle's bound for csr-vector and the
+    segmented sum."""
+    rng = np.random.default_rng(n)
+    m = 5000
+    lens = rng.integers(1, 17, m)
+    rp = np.zeros(m + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    col = rng.integers(0, n, int(rp[-1])).astype(np.int32)
+    for i in range(m):
+        col[rp[i]:rp[i + 1]] = np.sort(col[rp[i]:rp[i + 1]])
+    col[0] = 0
+    col[-1] = n - 1
+    c = cases.Case(f"read-ahead n={n}", m, n, rp.astype(np.int32), col,
+                   rng.standard_normal(int(rp[-1])), rng.standard_normal(n))
+    for variant, tpv, cfg in ((1, 8, (8, 4, 3)), (0, 0, (8, 1, 1)), (0, 0, (4, 4, 2))):
+        say(f"read-ahead n={n} {np.dtype(dtype).name} variant={variant} cfg={cfg}")
+        y, _ = gpu_spmv(c, dtype, cfg=cfg, variant=variant, tpv=tpv)
+        e, _ = rel_err(c, dtype, y)
+        assert e <= TOL[dtype], (n, variant, cfg, e)
+
+
 @pytest.mark.parametrize("name,dtype", [("uniform", np.float64), ("uniform", np.float32),
                                         ("power-law", np.float64), ("power-law", np.float32),
                                         ("laplace", np.float64), ("block3", np.float64)])
